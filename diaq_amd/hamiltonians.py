@@ -1,0 +1,109 @@
+"""Synthetic Hamiltonians and states built directly in diagonal (DIA) form.
+
+The reference has no bulk constructor: its Python API builds matrices from dense arrays (from_numpy,
+src/dense_to_diagonals.hpp:20-58) or element by element (set_at_diag, src/diaQMatrix.hpp:666-674), which cannot
+reach 2^20-2^24 rows. These builders produce the diagonals themselves, in ascending offset order (the order
+from_numpy would give), as ``(rows, cols, offsets, vals)`` with ``vals[k]`` the complex128 storage of diagonal
+``offsets[k]`` (position p holds A[p, p+d] for d >= 0 and A[p-d, p] for d < 0).
+
+Conventions: qubit i is bit i of the basis index; Z|0> = |0>, so z_i = 1 - 2*bit_i; open boundary.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _bits(n_qubits: int, N: int, i: int, idx=None) -> np.ndarray:
+    idx = np.arange(N, dtype=np.int64) if idx is None else idx
+    return (idx >> i) & 1
+
+
+def tfim(n_qubits: int, J: float = 1.0, h: float = 0.7):
+    """H = -J sum_{i<n-1} Z_i Z_{i+1} - h sum_i X_i: main diagonal plus offsets +-2^k (2n + 1 diagonals).
+
+    X_k maps |r> to |r xor 2^k>, so H[r, r + 2^k] = -h where bit k of r is 0 (offset +2^k at position p = r) and
+    H[r, r - 2^k] = -h where bit k of r is 1 (offset -2^k at position p = r - 2^k, whose bit k is 0): on both
+    diagonals the value at position p is -h exactly when bit k of p is 0.
+    """
+    N = 1 << n_qubits
+    idx = np.arange(N, dtype=np.int64)
+    z = 1.0 - 2.0 * np.stack([_bits(n_qubits, N, i, idx) for i in range(n_qubits)]).astype(np.float64)
+    diag = np.zeros(N)
+    for i in range(n_qubits - 1):
+        diag += -J * z[i] * z[i + 1]
+    diags = {0: diag.astype(np.complex128)}
+    for k in range(n_qubits):
+        L = N - (1 << k)
+        p = np.arange(L, dtype=np.int64)
+        v = np.where(((p >> k) & 1) == 0, -h, 0.0).astype(np.complex128)
+        diags[1 << k] = v
+        diags[-(1 << k)] = v.copy()
+    offs = sorted(diags)
+    return N, N, offs, [diags[d] for d in offs]
+
+
+def xxz(n_qubits: int, Jxy: float = 1.0, Jz: float = 0.8, fields=None, seed: int = 0):
+    """H = sum_{i<n-1} [Jxy (X_i X_{i+1} + Y_i Y_{i+1}) + Jz Z_i Z_{i+1}] + sum_i h_i Z_i, h_i ~ U[-1, 1] (seeded).
+
+    XX + YY = 2 (s+ s- + s- s+) swaps the bit pair (i, i+1) when the bits differ, with amplitude 2 Jxy; the 00 <-> 11
+    parts of XX and YY cancel, so only offsets +-2^i (i = 0..n-2) appear: H[r, r + 2^i] = 2 Jxy where bits (i, i+1)
+    of r are (1, 0), and H[r, r - 2^i] = 2 Jxy where they are (0, 1); at position p both diagonals hold 2 Jxy
+    exactly when bits (i, i+1) of p are (1, 0). 2(n-1) + 1 diagonals.
+    """
+    N = 1 << n_qubits
+    if fields is None:
+        fields = np.random.default_rng(seed).uniform(-1.0, 1.0, n_qubits)
+    fields = np.asarray(fields, np.float64)
+    idx = np.arange(N, dtype=np.int64)
+    z = 1.0 - 2.0 * np.stack([_bits(n_qubits, N, i, idx) for i in range(n_qubits)]).astype(np.float64)
+    diag = np.zeros(N)
+    for i in range(n_qubits - 1):
+        diag += Jz * z[i] * z[i + 1]
+    for i in range(n_qubits):
+        diag += fields[i] * z[i]
+    diags = {0: diag.astype(np.complex128)}
+    for i in range(n_qubits - 1):
+        L = N - (1 << i)
+        p = np.arange(L, dtype=np.int64)
+        v = np.where((((p >> i) & 1) == 1) & (((p >> (i + 1)) & 1) == 0), 2.0 * Jxy, 0.0).astype(np.complex128)
+        diags[1 << i] = v
+        diags[-(1 << i)] = v.copy()
+    offs = sorted(diags)
+    return N, N, offs, [diags[d] for d in offs]
+
+
+def random_offsets(N: int, n_diags: int, seed: int = 0) -> list:
+    """n_diags distinct integer offsets drawn uniformly from [-N/2, N/2] (seeded numpy default_rng), ascending."""
+    rng = np.random.default_rng(seed)
+    offs = set()
+    while len(offs) < n_diags:
+        offs.add(int(rng.integers(-N // 2, N // 2, endpoint=True)))
+    return sorted(offs)
+
+
+def product_state(n_qubits: int, seed: int = 0) -> np.ndarray:
+    """(x)_i (cos t_i |0> + e^{i p_i} sin t_i |1>) with t_i ~ U[0, pi), p_i ~ U[0, 2 pi) (seeded): unit norm."""
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0.0, np.pi, n_qubits)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n_qubits)
+    psi = np.ones(1, np.complex128)
+    for i in range(n_qubits):  # qubit i is bit i: the later kron factor is the lower bit
+        q = np.array([np.cos(th[i]), np.exp(1j * ph[i]) * np.sin(th[i])])
+        psi = np.kron(q, psi)
+    return psi
+
+
+def random_state(N: int, seed: int = 0) -> np.ndarray:
+    """Complex-normal vector divided by its 2-norm (seeded numpy default_rng)."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    return v / np.linalg.norm(v)
+
+
+def stored_elements(rows: int, cols: int, offsets) -> int:
+    """Sum of the diagonal lengths (src/diaQMatrix.hpp:390-409)."""
+    s = 0
+    for d in offsets:
+        n = rows + d if d < 0 else (cols - d if d > 0 else min(rows, cols))
+        s += max(0, min(n, rows, cols))
+    return s

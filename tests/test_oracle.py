@@ -1,0 +1,341 @@
+"""Pins for the CPU oracle: every oracle function is checked against something other than itself -- the reference's
+own fixtures (tests/golden/), dense numpy/scipy, closed forms, or brute force on tiny inputs."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import oracle as O
+from diaq_amd import hamiltonians as Hm
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rand_dia(rows, cols, offsets, rng, fill=1.0):
+    vals = []
+    for d in offsets:
+        n = O.diagonal_length(d, rows, cols)
+        v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        if fill < 1.0:
+            v[rng.random(n) > fill] = 0
+        vals.append(v)
+    return O.Dia(rows, cols, list(offsets), vals)
+
+
+def dec(e):
+    return (np.array(e["re"]) + 1j * np.array(e["im"])).reshape(e["shape"])
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+def test_diagonal_length_brute():
+    for rows, cols in [(5, 5), (3, 6), (6, 2), (1, 7), (9, 4)]:
+        I, J = np.indices((rows, cols))
+        for d in range(-(rows + 1), cols + 2):
+            assert O.diagonal_length(d, rows, cols) == int(np.sum(J - I == d)), (rows, cols, d)
+
+
+def test_padded_offsets_closed_form():
+    offs = [0, -1, 1, -5, 7]
+    starts, total = O.padded_offsets(offs, 20, 20)
+    assert all(s % 8 == 0 for s in starts)
+    for k in range(len(offs) - 1):
+        assert starts[k + 1] - starts[k] == 8 * math.ceil(O.diagonal_length(offs[k], 20, 20) / 8)
+    assert total == starts[-1] + O.diagonal_length(offs[-1], 20, 20)
+
+
+def test_symmetric_order():
+    assert O.symmetric_order(5) == [0, -1, 1, -2, 2]
+
+
+def test_dense_roundtrip_ascending():
+    rng = np.random.default_rng(1)
+    for shape in [(7, 7), (4, 9), (9, 4)]:
+        A = rng.standard_normal(shape) * (rng.random(shape) < 0.3)
+        M = O.from_dense(A)
+        assert M.offsets == sorted(M.offsets)
+        assert np.array_equal(O.to_dense(M), A)
+        for d in M.offsets:
+            assert np.any(np.diagonal(A, d) != 0)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("shape,offs", [((64, 64), [-3, -1, 0, 2, 5]), ((50, 70), [-9, 0, 7, 30]),
+                                        ((70, 50), [-40, -2, 0, 3]), ((33, 33), [4]), ((33, 33), [-32])])
+def test_spmv_vs_dense(shape, offs):
+    rng = np.random.default_rng(2)
+    M = rand_dia(*shape, offs, rng)
+    x = rng.standard_normal(shape[1]) + 1j * rng.standard_normal(shape[1])
+    np.testing.assert_allclose(O.spmv(M, x), O.to_dense(M) @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_spmv_known_answer_spmv2():
+    g = json.load(open(os.path.join(GOLD, "spmv_known.json")))
+    A = dec(g["A"])
+    M = O.from_dense(A)
+    assert M.offsets == g["printed_offsets"]
+    x = np.array(g["x"]["re"]) + 1j * np.array(g["x"]["im"])
+    y = O.spmv(M, x)
+    printed = np.array(g["y_printed"]["re"]) + 1j * np.array(g["y_printed"]["im"])
+    np.testing.assert_allclose(y, printed, atol=2e-6)  # six printed digits
+
+
+def test_spmv_identity_and_paulis():
+    n = 5
+    N = 1 << n
+    x = Hm.random_state(N, 3)
+    np.testing.assert_array_equal(O.spmv(O.identity(N), x), x)
+    # X on qubit 2 as a DIA matrix from the TFIM builder with J = 0, h = -1: sum_k X_k
+    rows, cols, offs, vals = Hm.tfim(n, J=0.0, h=-1.0)
+    M = O.Dia(rows, cols, offs, vals)
+    idx = np.arange(N)
+    expect = sum(x[idx ^ (1 << k)] for k in range(n))
+    np.testing.assert_allclose(O.spmv(M, x), expect, atol=1e-14)
+
+
+def test_spmv_ntimes_rules():
+    rng = np.random.default_rng(4)
+    M = rand_dia(16, 16, [-1, 0, 2], rng)
+    x = rng.standard_normal(16) + 0j
+    np.testing.assert_allclose(O.spmv_ntimes(M, x, 3), np.linalg.matrix_power(O.to_dense(M), 3) @ x, rtol=1e-12)
+    np.testing.assert_array_equal(O.spmv_ntimes(M, x, 0), x)
+    R = rand_dia(8, 16, [0, 3], rng)
+    with pytest.raises(RuntimeError):
+        O.spmv_ntimes(R, x, 2)
+    with pytest.raises(RuntimeError):
+        O.spmv_ntimes(M, x, -1)
+    np.testing.assert_array_equal(O.spmv_ntimes(O.Dia(16, 16, [], []), x, 2), np.zeros(16))
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+def test_spgemm_reference_known_answer_pairs():
+    g = json.load(open(os.path.join(GOLD, "spgemm_cases.json")))
+    assert len(g["cases"]) == 51
+    for c in g["cases"]:
+        A, B, P = dec(c["A"]), dec(c["B"]), dec(c["product"])
+        C = O.product_host(O.from_dense(A), O.from_dense(B))
+        assert C.offsets == c["product_offsets_pruned"], c["case"]
+        np.testing.assert_allclose(O.to_dense(C), P, rtol=1e-13, atol=1e-13)
+
+
+def test_dryrun_brute_force_structure():
+    rng = np.random.default_rng(5)
+    for rows, inner, cols in [(8, 8, 8), (6, 9, 5), (12, 5, 12)]:
+        oa = sorted(rng.choice(np.arange(-(rows - 1), inner), 4, replace=False).tolist())
+        ob = sorted(rng.choice(np.arange(-(inner - 1), cols), 3, replace=False).tolist())
+        A = rand_dia(rows, inner, oa, rng)
+        B = rand_dia(inner, cols, ob, rng)
+        offs, pairs = O.product_dryrun(A.offsets, B.offsets, rows, cols)
+        # brute force: which (a, b) slot pairs have a common index (i, k, j) in the dense product
+        P = np.abs(O.to_dense(A)) @ np.abs(O.to_dense(B))  # no cancellation with |.|
+        nz = [d for d in range(-(rows - 1), cols) if np.any(np.diagonal(P, d) != 0)]
+        assert set(nz) <= set(offs)
+        assert offs == sorted(offs)
+        for d, pl in zip(offs, pairs):
+            assert pl == sorted(pl)  # A-slot order, then B-slot order
+            for sa, sb in pl:
+                assert A.offsets[sa] + B.offsets[sb] == d
+            if d not in nz:  # a structural diagonal with no overlapping positions stays all zero
+                C = O.product(A, B)
+                assert not np.any(C.vals[C.offsets.index(d)])
+        total = sum(len(p) for p in pairs)
+        assert total == sum(1 for a in A.offsets for b in B.offsets if -(rows - 1) <= a + b <= cols - 1)
+
+
+def test_prune_constructed_cancellation():
+    A = O.from_dense(np.array([[1, 1], [0, 1]], complex))
+    B = O.from_dense(np.array([[1, -1], [0, 1]], complex))
+    C = O.product(A, B)
+    assert C.offsets == [0, 1]
+    assert O.prune(C).offsets == [0]
+    np.testing.assert_array_equal(O.to_dense(O.prune(C)), np.eye(2))
+
+
+def test_tfim20_hh_dryrun_closed_form():
+    offs = [0] + [s * (1 << k) for k in range(20) for s in (1, -1)]
+    N = 1 << 20
+    res, pairs = O.product_dryrun(sorted(offs), sorted(offs), N, N)
+    assert len(res) == 727
+    # 41 x 41 = 1681 pairs, of which +-2^19 + +-2^19 = +-2^20 fall outside [-(N-1), N-1]
+    assert sum(len(p) for p in pairs) == 41 * 41 - 2
+    assert Hm.stored_elements(N, N, res) == 687_341_554
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+def test_union_add_norms_trace():
+    rng = np.random.default_rng(6)
+    A = rand_dia(20, 20, [-2, 0, 5], rng)
+    B = rand_dia(20, 20, [-7, 0, 1, 5], rng)
+    C = O.add_union(A, B)
+    assert C.offsets == [-7, -2, 0, 1, 5]
+    np.testing.assert_allclose(O.to_dense(C), O.to_dense(A) + O.to_dense(B), rtol=1e-15)
+    assert O.frobenius(A) == pytest.approx(np.linalg.norm(O.to_dense(A)), rel=1e-14)
+    assert O.trace(A) == pytest.approx(np.trace(O.to_dense(A)), rel=1e-14)
+    D = O.to_dense(A)
+    assert O.estimate_1norm(A) == pytest.approx(np.max(np.sum(np.abs(D.real) + np.abs(D.imag), axis=0)), rel=1e-14)
+
+
+def test_tfim12_norm_and_scaling_closed_form():
+    rows, cols, offs, vals = Hm.tfim(12, J=1.0, h=0.7)
+    M = O.Dia(rows, cols, offs, vals)
+    assert len(offs) == 25 and M.stored == 94_210
+    assert O.estimate_1norm(M) == pytest.approx(11 * 1.0 + 12 * 0.7, rel=1e-14)  # 19.4
+    assert O.scaling_exponent(O.estimate_1norm(M), 0.1, 1.0) == 1
+
+
+def test_xxz20_structure():
+    n = 20
+    N = 1 << n
+    offs = [0] + [s * (1 << i) for i in range(n - 1) for s in (1, -1)]
+    assert len(offs) == 39 and Hm.stored_elements(N, N, offs) == 39_845_890
+
+
+@pytest.mark.parametrize("builder", ["tfim", "xxz"])
+def test_hamiltonian_builders_vs_kron(builder):
+    n = 5
+    X = np.array([[0, 1], [1, 0]], complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0, -1.0]).astype(complex)
+
+    def op(single, i):  # qubit i is bit i: kron order puts qubit n-1 first
+        mats = [np.eye(2)] * n
+        mats = list(mats)
+        mats[n - 1 - i] = single
+        out = np.ones((1, 1))
+        for m in mats:
+            out = np.kron(out, m)
+        return out
+
+    if builder == "tfim":
+        J, h = 1.3, 0.7
+        H = sum(-J * op(Z, i) @ op(Z, i + 1) for i in range(n - 1)) + sum(-h * op(X, i) for i in range(n))
+        r, c, offs, vals = Hm.tfim(n, J, h)
+    else:
+        f = np.random.default_rng(0).uniform(-1, 1, n)
+        H = sum(1.0 * (op(X, i) @ op(X, i + 1) + op(Y, i) @ op(Y, i + 1)) + 0.8 * op(Z, i) @ op(Z, i + 1)
+                for i in range(n - 1)) + sum(f[i] * op(Z, i) for i in range(n))
+        r, c, offs, vals = Hm.xxz(n, 1.0, 0.8, fields=f)
+    M = O.Dia(r, c, offs, vals)
+    np.testing.assert_allclose(O.to_dense(M), H, atol=1e-14)
+    assert O.from_dense(H).offsets == offs
+
+
+def test_product_state_matches_kron():
+    n = 4
+    psi = Hm.product_state(n, 7)
+    assert np.linalg.norm(psi) == pytest.approx(1.0, rel=1e-14)
+    rng = np.random.default_rng(7)
+    th = rng.uniform(0, np.pi, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    for idx in range(1 << n):
+        amp = 1.0 + 0j
+        for i in range(n):
+            amp *= np.cos(th[i]) if ((idx >> i) & 1) == 0 else np.exp(1j * ph[i]) * np.sin(th[i])
+        assert psi[idx] == pytest.approx(amp, abs=1e-15)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+def test_pow_vs_matrix_power():
+    rng = np.random.default_rng(8)
+    A = rand_dia(48, 48, [-1, 0, 1], rng)
+    A = O.Dia(48, 48, A.offsets, [v / 4 for v in A.vals])
+    D = O.to_dense(A)
+    for n in [0, 1, 2, 3, 7, 8]:
+        np.testing.assert_allclose(O.to_dense(O.pow_gpu(A, n)), np.linalg.matrix_power(D, n), rtol=1e-12,
+                                   atol=1e-14)
+
+
+def test_expm_negative_vs_scipy_tfim6():
+    rows, cols, offs, vals = Hm.tfim(6, 1.0, 0.7)
+    M = O.Dia(rows, cols, offs, vals)
+    for t in [0.05, 0.1, 0.7]:
+        U = O.expm_negative(M, t)
+        ref = scipy.linalg.expm(-1j * t * O.to_dense(M))
+        np.testing.assert_allclose(O.to_dense(U), ref, atol=1e-11)
+
+
+def test_expm_vs_scipy_and_single_diag_quirk():
+    rng = np.random.default_rng(9)
+    A = rand_dia(24, 24, [-2, 0, 3], rng)
+    A = O.Dia(24, 24, A.offsets, [v * 0.3 for v in A.vals])
+    np.testing.assert_allclose(O.to_dense(O.expm(A)), scipy.linalg.expm(O.to_dense(A)), rtol=1e-10, atol=1e-12)
+    # one main diagonal: elementwise exp, a closed form
+    D = rand_dia(10, 10, [0], rng)
+    np.testing.assert_allclose(O.to_dense(O.expm_negative(D, 0.3)), np.diag(np.exp(-1j * 0.3 * D.vals[0])),
+                               rtol=1e-14)
+    np.testing.assert_allclose(O.to_dense(O.expm(D)), np.diag(np.exp(D.vals[0])), rtol=1e-14)
+    # one off-diagonal: the reference writes it onto the main diagonal (kept quirk)
+    S = rand_dia(10, 10, [3], rng)
+    U = O.to_dense(O.expm_negative(S, 0.5))
+    exp_diag = np.ones(10, complex)
+    exp_diag[:7] = np.exp(-1j * 0.5 * S.vals[0])
+    np.testing.assert_allclose(U, np.diag(exp_diag), rtol=1e-14)
+
+
+def test_taylor_action_and_hamsim():
+    rows, cols, offs, vals = Hm.xxz(6, seed=1)
+    H = O.Dia(rows, cols, offs, vals)
+    psi = Hm.product_state(6, 2)
+    out = O.hamiltonian_simulation(H, psi, t=1.0, r=10, max_iterations=30)
+    ref = scipy.linalg.expm(-1j * 1.0 * O.to_dense(H)) @ psi
+    np.testing.assert_allclose(out, ref, atol=1e-12)
+    assert np.linalg.norm(out) == pytest.approx(1.0, abs=1e-12)
+    np.testing.assert_array_equal(O.hamiltonian_simulation(H, psi, 1.0, 0), psi)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("M,K,R,offs", [(40, 40, 5, [-3, -1, 0, 2, 5]), (30, 36, 1, [0, 7, -9]),
+                                        (20, 14, 3, list(range(-13, 14, 3)))])
+def test_spmm_dia_vs_dense(M, K, R, offs):
+    rng = np.random.default_rng(10)
+    V = rng.standard_normal((len(offs), M)) + 1j * rng.standard_normal((len(offs), M))
+    A = np.zeros((M, K), complex)
+    r = np.arange(M)
+    for k, o in enumerate(offs):
+        ok = (r + o >= 0) & (r + o < K)
+        V[k, ~ok] = 0
+        A[r[ok], r[ok] + o] = V[k, ok]
+    B = rng.standard_normal((K, R)) + 1j * rng.standard_normal((K, R))
+    np.testing.assert_allclose(O.spmm_dia(offs, V, M, K, B), A @ B, rtol=1e-13, atol=1e-13)
+
+
+def test_dia_viability_cases():
+    v = O.dia_viability(4096, 4096, 5, 5 * 4096, 1)
+    assert v["viable"] and v["diagonal_budget"] == 128
+    v = O.dia_viability(4096, 4096, 200, 200 * 4096, 1)
+    assert not v["diagonals_ok"] and v["reason"].startswith("diagonals")
+    v = O.dia_viability(4096, 4096, 9, 9 * 4096, 64)
+    assert v["diagonal_budget"] == 8 and not v["viable"]
+    v = O.dia_viability(1000, 1000, 4, 400, 1)
+    assert not v["storage_ok"]
+    # dia bytes 16*5*4096 + 40 vs csr 24*nnz + 8*4097: 0.9 threshold by hand
+    assert v["dia_bytes"] == 16 * 4 * 1000 + 32 and v["csr_bytes"] == 24 * 400 + 8 * 1001
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+def test_c_oracle_spmv_matches_numpy():
+    from oracle import coracle
+    rng = np.random.default_rng(11)
+    for shape, offs in [((1000, 1000), [-300, -1, 0, 1, 17, 999]), ((500, 800), [-499, 0, 250, 799]),
+                        ((800, 500), [-799, -10, 0, 499])]:
+        M = rand_dia(*shape, offs, rng)
+        x = rng.standard_normal(shape[1]) + 1j * rng.standard_normal(shape[1])
+        o, st, re, im = coracle.packed(M.offsets, M.vals)
+        ref = O.spmv(M, x)
+        np.testing.assert_allclose(coracle.spmv_rows(shape[0], shape[1], o, st, re, im, x), ref, rtol=1e-13,
+                                   atol=1e-13)
+        yr, yi = np.empty(shape[0]), np.empty(shape[0])
+        coracle.spmv_tasks_split(shape[0], shape[1], o, st, re, im, np.ascontiguousarray(x.real),
+                                 np.ascontiguousarray(x.imag), yr, yi, block_size=64, n_threads=3)
+        np.testing.assert_allclose(yr + 1j * yi, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_c_oracle_block_size_rule():
+    from oracle import coracle
+    b = coracle.block_size()
+    assert b % 8 == 0 and b > 0
+    l1 = os.sysconf("SC_LEVEL1_DCACHE_SIZE") if "SC_LEVEL1_DCACHE_SIZE" in os.sysconf_names else 0
+    if l1 == 49152:
+        assert b == 968  # floor(floor(49152 * 0.95 / 48) / 8) * 8
