@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the DiaQ GPU path on one MI355X.
+
+Headline (one JSON line on stdout): DIA SpMV y = A x at N = 2^24 with 64 diagonals at seeded uniform-random offsets
+in [-N/2, N/2] (numpy default_rng(0)), complex128 split real/imaginary values drawn on the device (torch, seed 0),
+everything resident in HBM. A step is one SpMV; value = compulsory bytes (16*S + 16*N + 16*N, S = stored elements)
+per second of the K timed steps, in GB/s; roofline.frac = that over the MI355X HBM peak (8 TB/s).
+
+Also reported: the reference-style CPU SpMV (the C oracle's restatement of the reference's OpenMP/SIMD path) timed on
+this host on the same matrix, and the time evolution of the 20-qubit XXZ chain (100 steps, dt = 0.05) by the plain
+FP64 fused path.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-evolution]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s vs HBM peak; evolutions/s batched-fp32 vs plain fp64 at 1e-6"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def spmv_section(args, torch, d, capi, Hm):
+    N = 1 << 24
+    offs = Hm.random_offsets(N, 64, seed=0)
+    lens = [Hm.stored_elements(N, N, [o]) for o in offs]
+    S = int(sum(lens))
+    log(f"[bench] spmv N=2^24 diagonals=64 S={S}")
+    L = capi.lib()
+    import ctypes
+    out = ctypes.c_void_p()
+    o32 = np.asarray(offs, np.int32)
+    capi.check(L.dq_devmat_zeros(N, N, len(offs), capi.ptr(o32, ctypes.c_int32), capi.DQ_F64, ctypes.byref(out)))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    re = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+    im = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+    capi.check(L.dq_devmat_set_packed_device(out, re.data_ptr(), im.data_ptr(), 0))
+    xr = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    xi = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    sid = stream.cuda_stream
+
+    def step():
+        capi.check(L.dq_spmv_device(out, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    bytes_alg = 16 * S + 16 * N + 16 * N
+    gbs = bytes_alg / (ms * 1e-3) / 1e9
+    log(f"[bench] spmv fp64: {ms:.3f} ms/step  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS:.3f} of peak)  wall {wall:.2f}s")
+    res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg)
+
+    # FP32 form of the same matrix (precision form): half the bytes
+    try:
+        out32 = ctypes.c_void_p()
+        capi.check(L.dq_devmat_convert(out, capi.DQ_F32, ctypes.byref(out32)))
+        xr32, xi32 = xr.float(), xi.float()
+        yr32, yi32 = torch.empty_like(xr32), torch.empty_like(xi32)
+        for _ in range(args.warmup):
+            capi.check(L.dq_spmv_device(out32, xr32.data_ptr(), xi32.data_ptr(), yr32.data_ptr(), yi32.data_ptr(), sid))
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            capi.check(L.dq_spmv_device(out32, xr32.data_ptr(), xi32.data_ptr(), yr32.data_ptr(), yi32.data_ptr(), sid))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms32 = e0.elapsed_time(e1) / args.steps
+        b32 = 8 * S + 8 * N + 8 * N
+        y64 = torch.complex(yr, yi)
+        err32 = float(torch.linalg.vector_norm(torch.complex(yr32.double(), yi32.double()) - y64) /
+                      torch.linalg.vector_norm(y64))
+        res["fp32"] = dict(ms=ms32, gbs=b32 / (ms32 * 1e-3) / 1e9, rel_l2_err_vs_fp64=err32)
+        log(f"[bench] spmv fp32: {ms32:.3f} ms/step  {res['fp32']['gbs']:.1f} GB/s  rel err vs fp64 {err32:.1e}")
+        L.dq_devmat_destroy(out32)
+        del xr32, xi32, yr32, yi32
+    except Exception as e:  # the headline stays valid without the FP32 figure
+        res["fp32"] = dict(error=str(e))
+
+    if not args.no_cpu:
+        try:
+            res["cpu"] = cpu_baseline(args, torch, re, im, lens, offs, xr, xi, yr, yi, N)
+        except Exception as e:
+            res["cpu"] = dict(error=str(e))
+            log(f"[bench] cpu baseline failed: {e}")
+    del re, im
+    L.dq_devmat_destroy(out)
+    torch.cuda.synchronize()
+    return res
+
+
+def cpu_baseline(args, torch, re, im, lens, offs, xr, xi, yr, yi, N):
+    """The reference's CPU SpMV as restated by the C oracle (2-D diagonal x tile tasks, thread-local y, merge), on
+    this host's cores, on the same matrix and x as the GPU run."""
+    from oracle import coracle
+    nt = cpu_threads()
+    log(f"[bench] cpu baseline: copying the matrix to the host, {nt} threads")
+    h_re = re.cpu().numpy()
+    h_im = im.cpu().numpy()
+    starts = np.zeros(len(lens), np.int64)
+    starts[1:] = np.cumsum(lens)[:-1]
+    o32 = np.asarray(offs, np.int32)
+    h_xr, h_xi = xr.cpu().numpy(), xi.cpu().numpy()
+    o_yr, o_yi = np.empty(N), np.empty(N)
+    coracle.spmv_tasks_split(N, N, o32, starts, h_re, h_im, h_xr, h_xi, o_yr, o_yi, 0, nt)  # warmup
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        coracle.spmv_tasks_split(N, N, o32, starts, h_re, h_im, h_xr, h_xi, o_yr, o_yi, 0, nt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    S = int(sum(lens))
+    gbs = (16 * S + 32 * N) / t / 1e9
+    g = torch.complex(yr, yi).cpu().numpy()
+    c = o_yr + 1j * o_yi
+    agree = float(np.max(np.abs(g - c)) / np.max(np.abs(c)))
+    log(f"[bench] cpu baseline: {t * 1e3:.1f} ms/SpMV  {gbs:.1f} GB/s  GPU-vs-CPU max rel diff {agree:.1e}")
+    return dict(value=gbs, unit="GB/s", cores=nt, kind="port", ms=t * 1e3, block_size=coracle.block_size(),
+                gpu_vs_cpu_max_rel_diff=agree,
+                sample="the full N=2^24, 64-diagonal SpMV (same matrix and x as the GPU run), median of 5 after 1 warmup")
+
+
+def norm_rule_m(norm1_dt: float, tol: float, m_max: int = 100) -> int:
+    """Smallest m with (||H dt||_1)^(m+1) / (m+1)! <= tol (the plain norm rule for a fixed-degree Taylor step)."""
+    for m in range(1, m_max + 1):
+        if (m + 1) * math.log(norm1_dt) - math.lgamma(m + 2) <= math.log(tol):
+            return m
+    return m_max
+
+
+def evolution_section(args, torch, d, capi, Hm):
+    n = 20
+    rows, cols, offs, vals = Hm.xxz(n, 1.0, 0.8, seed=0)
+    H = d.to_device(d.diaQMatrix.from_diagonals([rows, cols], offs, vals))
+    psi0 = Hm.product_state(n, seed=0)
+    steps, dt = 100, 0.05
+    norm1 = d.estimate_1norm(H)
+    tol_step = 1e-6 / steps  # per-step truncation budget for a 1e-6 total
+    m_plain = norm_rule_m(norm1 * dt, tol_step)
+    out = dict(workload=f"XXZ-{n} (Jxy=1, Jz=0.8, h_i~U[-1,1] seed 0), {steps} steps of dt={dt}, product state seed 0",
+               norm1=norm1, m_plain=m_plain, tol_step=tol_step)
+    sid = torch.cuda.current_stream().cuda_stream
+    for label, m in (("plain_fp64", m_plain), ("reference_m100", 100)):
+        pr = torch.as_tensor(psi0.real, device="cuda").contiguous()
+        pi = torch.as_tensor(psi0.imag, device="cuda").contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.hamsim_device(H, pr.data_ptr(), pi.data_ptr(), steps * dt, steps, m, sid)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out[label] = dict(m=m, seconds=t, evolutions_per_s=1.0 / t,
+                          norm=float(torch.sqrt(torch.sum(pr * pr + pi * pi))))
+        log(f"[bench] evolution {label}: m={m} {t:.3f} s/evolution")
+        if label == "plain_fp64":
+            ref_state = torch.complex(pr, pi)
+        else:
+            out["plain_vs_m100_l2"] = float(torch.linalg.vector_norm(torch.complex(pr, pi) - ref_state))
+    S = H.stored
+    out["bytes_per_term"] = 16 * S + 64 * rows
+    out["plain_fp64"]["gbs"] = out["bytes_per_term"] * m_plain * steps / out["plain_fp64"]["seconds"] / 1e9
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-evolution", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
+                 "run with --gpus 1")
+    import torch
+    import diaq_amd  # noqa: F401
+    import diaq_gpu as d
+    from diaq_amd import capi
+    from diaq_amd import hamiltonians as Hm
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    torch.cuda.set_device(0)
+    sp = spmv_section(args, torch, d, capi, Hm)
+    evo = None
+    if not args.no_evolution:
+        try:
+            evo = evolution_section(args, torch, d, capi, Hm)
+        except Exception as e:
+            evo = dict(error=str(e))
+            log(f"[bench] evolution failed: {e}")
+    cpu = sp.get("cpu", {})
+    line = {
+        "metric": METRIC,
+        "value": round(sp["gbs"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sp["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "DIA SpMV y=Ax, N=2^24, 64 diagonals at uniform-random offsets in [-N/2, N/2] "
+                               "(numpy default_rng seed 0), complex128 split re/im, values and x torch.randn seed 0",
+                   "N": sp["N"], "n_diags": 64, "stored_elements": sp["S"], "compulsory_bytes": sp["bytes"]},
+        "roofline": {"bound": "hbm", "achieved": round(sp["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
+                         else None),
+        "spmv_fp32": sp.get("fp32"),
+        "cpu_detail": cpu or None,
+        "evolution": evo,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,20 @@
+"""DiaQ GPU path on MI355X (gfx950): diagonal-format sparse linear algebra with hand-written HIP kernels.
+
+``diaq_amd.diaq_gpu`` is the drop-in for the reference's ``diaq_gpu`` module (importing this package also makes
+``import diaq_gpu`` work); ``diaq_amd.capi`` is the ctypes view of the C ABI (include/diaq_hip.h). Both need the
+native libraries built in-tree (``python -m diaq_amd.build``): there is no CPU fallback, a missing library is an
+ImportError.
+"""
+import os
+import sys
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+try:
+    from . import diaq_gpu  # noqa: F401  (links libdiaq_hip.so next to it)
+except ImportError as e:  # pragma: no cover - depends on the build
+    raise ImportError(f"diaq_amd: native module not built or not loadable ({e}); run python -m diaq_amd.build") from e
+
+sys.modules.setdefault("diaq_gpu", diaq_gpu)
+
+LIBRARY = os.path.join(_HERE, "libdiaq_hip.so")

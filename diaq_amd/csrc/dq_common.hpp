@@ -1,0 +1,176 @@
+// Internals shared by the DiaQ HIP library (gfx950): error handling, the device DIA layout, allocation, launch helpers.
+//
+// Device layout of a DIA matrix (dq_devmat):
+//   - values in split real / imaginary arrays (structure of arrays), FP64 or FP32;
+//   - diagonal k (offset d) covers rows [rlo, rhi) with rlo = max(0, -d) and rhi = rlo + len, where len is the
+//     reference's diagonal length (src/deviceMatrix_gpu.cu:16-28); its element for row r, A[r, r + d], lives at index
+//     rowbase + r. Each diagonal's storage start is padded so that rowbase is a multiple of DQ_ALIGN elements: a wave
+//     working on 64 consecutive rows then reads whole, aligned 128-byte lines of every diagonal. Padding is zero.
+//   - metadata (offset, rowbase, row range) in a device array of DiagMeta, plus a host copy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "diaq_hip.h"
+
+namespace dq {
+
+// ------------------------------------------------------------------------------------------------------------------
+// Errors: every C entry point runs its body under guard(), which turns exceptions into a status code plus a
+// thread-local message (dq_last_error).
+// ------------------------------------------------------------------------------------------------------------------
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct AllocError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void set_last_error(const std::string &msg);
+
+template <typename F>
+int guard(F &&f) {
+    try {
+        f();
+        return DQ_OK;
+    } catch (const ArgError &e) {
+        set_last_error(e.what());
+        return DQ_ERR_ARG;
+    } catch (const AllocError &e) {
+        set_last_error(e.what());
+        return DQ_ERR_ALLOC;
+    } catch (const HipError &e) {
+        set_last_error(e.what());
+        return DQ_ERR_HIP;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return DQ_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return DQ_ERR_STATE;
+    }
+}
+
+#define DQ_HIP(call)                                                                                               \
+    do {                                                                                                           \
+        hipError_t dq_err_ = (call);                                                                               \
+        if (dq_err_ != hipSuccess) {                                                                               \
+            (void)hipGetLastError();                                                                               \
+            throw ::dq::HipError(std::string("HIP error: ") + hipGetErrorString(dq_err_) + " (" + __FILE__ + ":" + \
+                                 std::to_string(__LINE__) + ")");                                                  \
+        }                                                                                                          \
+    } while (0)
+
+#define DQ_LAUNCH_CHECK() DQ_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(uintptr_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------------------------------------------------------
+// Device memory
+// ------------------------------------------------------------------------------------------------------------------
+void *dev_alloc(size_t bytes);          // hipMalloc with a clear error; nullptr for 0 bytes
+void dev_free(void *p);                 // hipFree, never throws
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) : p(dev_alloc(n)), bytes(n) {}
+    ~DevBuf() { dev_free(p); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { dev_free(p); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ------------------------------------------------------------------------------------------------------------------
+// DIA layout
+// ------------------------------------------------------------------------------------------------------------------
+constexpr int64_t DQ_ALIGN_BYTES = 128;
+
+struct DiagMeta {
+    int64_t rowbase;  // element index of row 0 on this diagonal (row r at rowbase + r)
+    int64_t rlo;      // first row holding an element
+    int64_t rhi;      // one past the last row
+    int32_t off;      // diagonal offset d: row r meets column r + d
+    int32_t pad_;
+};
+static_assert(sizeof(DiagMeta) == 32, "DiagMeta is 32 bytes");
+
+// Length of diagonal d of a rows x cols matrix (src/deviceMatrix_gpu.cu:16-28).
+inline int64_t diagonal_length(int32_t d, int64_t rows, int64_t cols) {
+    int64_t n = d < 0 ? rows + d : (d > 0 ? cols - d : (rows < cols ? rows : cols));
+    if (n > rows) n = rows;
+    if (n > cols) n = cols;
+    return n < 0 ? 0 : n;
+}
+
+inline int elem_bytes(int precision) { return precision == DQ_F32 ? 4 : 8; }
+
+}  // namespace dq
+
+// The opaque handle of the C ABI.
+struct dq_devmat {
+    uint32_t rows = 0, cols = 0;
+    int precision = DQ_F64;
+    std::vector<int32_t> offs;        // diagonal offsets in slot order (the caller's order is kept)
+    std::vector<int64_t> start;       // storage start of each diagonal (elements)
+    std::vector<int64_t> len;         // stored elements of each diagonal
+    std::vector<dq::DiagMeta> meta;   // host copy of the device metadata
+    int64_t total = 0;                // allocated elements per value array (padding included)
+    dq::DevBuf re, im;                // values
+    dq::DevBuf d_meta;                // DiagMeta[n]
+
+    int n() const { return static_cast<int>(offs.size()); }
+    int64_t stored() const {
+        int64_t s = 0;
+        for (int64_t l : len) s += l;
+        return s;
+    }
+};
+
+namespace dq {
+
+// Compute the layout of (rows, cols, offsets) and allocate zeroed values and device metadata (stream-ordered).
+std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32_t *offsets, int n, int precision,
+                                       hipStream_t stream);
+
+// Launch geometry helpers.
+inline unsigned blocks_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
+    int64_t b = (work + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return static_cast<unsigned>(b);
+}
+
+// Elementwise kernels over the stored values of matrices that share one layout (same shape and offsets).
+void fill_identity(dq_devmat &M, hipStream_t s);  // M must have the single diagonal 0
+void scale_real(dq_devmat &M, double factor, hipStream_t s);
+void times_i_scale_copy(const dq_devmat &A, dq_devmat &C, double imag, hipStream_t s);  // C = (i*imag) A
+// A + B over the ascending union of their diagonals (src/diaQMatrix.hpp:867-916); shapes must match.
+std::unique_ptr<dq_devmat> union_add(const dq_devmat &A, const dq_devmat &B, hipStream_t s);
+double frobenius(const dq_devmat &A, hipStream_t s);
+double estimate_1norm(const dq_devmat &A, hipStream_t s);
+std::unique_ptr<dq_devmat> prune(const dq_devmat &A, hipStream_t s);
+std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
+
+// C = (A B) / divisor with the structural dry-run (product.hip).
+std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);
+
+// y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
+void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);
+
+}  // namespace dq

@@ -1,0 +1,614 @@
+// diaq_gpu: the Python module of the MI355X DiaQ GPU path, a drop-in for the reference's diaq_gpu module
+// (include/pythonWrappers.cpp:31-35, GPU block :589-654, shared entries :41-587).
+//
+// Host C++ only: the host DIA container, host vectors and conversions live here; every GPU operation goes through the
+// C ABI of libdiaq_hip.so (include/diaq_hip.h). C ABI failures are rethrown as RuntimeError with the library's
+// message, as the reference's std::runtime_error are (pythonWrappers.cpp; src/cuda_check.hpp:11-18).
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "diaq_hip.h"
+
+namespace py = pybind11;
+using cplx = std::complex<double>;
+
+static void check(int rc) {
+    if (rc != DQ_OK) throw std::runtime_error(dq_last_error());
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Host DIA container (src/diaQMatrix.hpp:43-63): split real / imaginary storage with every diagonal start rounded up
+// to 8 elements (the reference's AVX-512 build: initStorage, :290-357), so diagOffsets / totalLength read the same.
+// ---------------------------------------------------------------------------------------------------------------------
+static constexpr unsigned VEC_WIDTH = 8;
+
+static unsigned diag_len(int d, const std::array<unsigned, 2> &s) {
+    long long n = d < 0 ? (long long)s[0] + d : (d > 0 ? (long long)s[1] - d : std::min(s[0], s[1]));
+    if (n > s[0]) n = s[0];
+    if (n > s[1]) n = s[1];
+    return n < 0 ? 0u : (unsigned)n;
+}
+
+static std::vector<int> symmetric_order(unsigned n) {  // src/diaQMatrix.hpp:25-39
+    std::vector<int> o;
+    for (unsigned i = 0; i < n; ++i) o.push_back(i == 0 ? 0 : (i % 2 == 1 ? -(int)((i + 1) / 2) : (int)(i / 2)));
+    return o;
+}
+
+struct HostDia {
+    std::array<unsigned, 2> matrix_shape{0, 0};
+    std::vector<unsigned> shape;
+    std::vector<int> dIndices;
+    std::vector<unsigned> diagOffsets;
+    unsigned totalLength = 0;
+    std::vector<double> re, im;
+
+    HostDia() = default;
+    HostDia(std::array<unsigned, 2> ms, std::vector<int> d, std::vector<unsigned> ts)
+        : matrix_shape(ms), shape(std::move(ts)), dIndices(std::move(d)) {
+        init_storage();
+    }
+    virtual ~HostDia() = default;
+
+    void init_storage() {
+        diagOffsets.resize(dIndices.size());
+        unsigned total = 0;
+        for (size_t k = 0; k < dIndices.size(); ++k) {
+            if (total % VEC_WIDTH) total += VEC_WIDTH - total % VEC_WIDTH;
+            diagOffsets[k] = total;
+            total += diag_len(dIndices[k], matrix_shape);
+        }
+        totalLength = total;
+        re.assign(total, 0.0);
+        im.assign(total, 0.0);
+    }
+    unsigned len(size_t k) const { return diag_len(dIndices[k], matrix_shape); }
+    int slot(int d) const {
+        for (size_t k = 0; k < dIndices.size(); ++k)
+            if (dIndices[k] == d) return (int)k;
+        return -1;
+    }
+    std::shared_ptr<HostDia> copy() const { return std::make_shared<HostDia>(*this); }
+};
+
+struct HostIdentity : HostDia {  // src/specialMatrices.hpp:9-24
+    explicit HostIdentity(unsigned s) : HostDia({s, s}, {0}, {s, s}) {
+        std::fill(re.begin(), re.begin() + s, 1.0);
+    }
+};
+
+struct HostVector {  // src/vector.hpp:8-299 (the parts the GPU path uses)
+    unsigned numRows = 0;
+    std::vector<double> re, im;
+    explicit HostVector(unsigned n) : numRows(n), re(n, 0.0), im(n, 0.0) {}
+};
+
+// Shared shape / tensor-shape checks of the bindings.
+static std::array<unsigned, 2> pair_of(const std::vector<unsigned> &s) {
+    if (s.size() != 2) throw std::invalid_argument("a matrix shape needs exactly 2 entries");
+    return {s[0], s[1]};
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Device matrices (DeviceMatrix, src/deviceMatrix_gpu.hpp:37-68): an owning handle around the C ABI object.
+// ---------------------------------------------------------------------------------------------------------------------
+struct DeviceMatrix {
+    std::shared_ptr<dq_devmat> h;
+    std::vector<unsigned> tensor_shape;
+    explicit DeviceMatrix(dq_devmat *p, std::vector<unsigned> ts = {})
+        : h(p, dq_devmat_destroy), tensor_shape(std::move(ts)) {}
+    std::array<unsigned, 2> matrix_shape() const {
+        uint32_t r, c;
+        check(dq_devmat_info(h.get(), &r, &c, nullptr, nullptr, nullptr));
+        return {r, c};
+    }
+    int n_diags() const {
+        int32_t n;
+        check(dq_devmat_info(h.get(), nullptr, nullptr, &n, nullptr, nullptr));
+        return n;
+    }
+    int precision() const {
+        int p;
+        check(dq_devmat_info(h.get(), nullptr, nullptr, nullptr, &p, nullptr));
+        return p;
+    }
+    std::vector<int> diagonals() const {
+        std::vector<int32_t> o(n_diags());
+        check(dq_devmat_offsets(h.get(), o.data()));
+        return std::vector<int>(o.begin(), o.end());
+    }
+};
+
+static int parse_precision(const std::string &p) {
+    if (p == "fp64" || p == "float64" || p == "complex128") return DQ_F64;
+    if (p == "fp32" || p == "float32" || p == "complex64") return DQ_F32;
+    throw std::invalid_argument("precision must be 'fp64' or 'fp32'");
+}
+
+static std::shared_ptr<DeviceMatrix> upload(const HostDia &A, int precision = DQ_F64) {
+    std::vector<int32_t> offs(A.dIndices.begin(), A.dIndices.end());
+    std::vector<int64_t> starts(A.diagOffsets.begin(), A.diagOffsets.end());
+    dq_devmat *p = nullptr;
+    check(dq_devmat_upload(A.matrix_shape[0], A.matrix_shape[1], (int32_t)offs.size(), offs.data(), starts.data(),
+                           A.re.data(), A.im.data(), precision, &p));
+    return std::make_shared<DeviceMatrix>(p, A.shape);
+}
+
+static std::shared_ptr<HostDia> download(const DeviceMatrix &D) {
+    auto ms = D.matrix_shape();
+    std::vector<unsigned> ts = D.tensor_shape;
+    if (ts.empty()) ts = {ms[0], ms[1]};
+    auto H = std::make_shared<HostDia>(ms, D.diagonals(), ts);
+    std::vector<int64_t> starts(H->diagOffsets.begin(), H->diagOffsets.end());
+    check(dq_devmat_download(D.h.get(), starts.data(), H->re.data(), H->im.data()));
+    return H;
+}
+
+static std::shared_ptr<DeviceMatrix> wrap(dq_devmat *p, std::vector<unsigned> ts = {}) {
+    return std::make_shared<DeviceMatrix>(p, std::move(ts));
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Conversions (src/conversions.cpp:70-212, src/dense_to_diagonals.hpp:20-58)
+// ---------------------------------------------------------------------------------------------------------------------
+static std::shared_ptr<HostDia> diagonals_from_dense(std::array<unsigned, 2> ms, const std::vector<unsigned> &ts,
+                                                     const cplx *v) {
+    const long long rows = ms[0], cols = ms[1];
+    std::vector<int> diags;
+    for (long long d = -(rows - 1); d < cols; ++d) {
+        const long long r0 = d >= 0 ? 0 : -d, c0 = d >= 0 ? d : 0;
+        const long long n = std::min(rows - r0, cols - c0);
+        for (long long j = 0; j < n; ++j) {
+            const cplx z = v[(r0 + j) * cols + c0 + j];
+            if (z.real() != 0.0 || z.imag() != 0.0) { diags.push_back((int)d); break; }
+        }
+    }
+    auto M = std::make_shared<HostDia>(ms, diags, ts);
+    for (size_t k = 0; k < diags.size(); ++k) {
+        const long long d = diags[k], r0 = d >= 0 ? 0 : -d, c0 = d >= 0 ? d : 0;
+        const long long n = std::min(rows - r0, cols - c0);
+        for (long long j = 0; j < n; ++j) {
+            const cplx z = v[(r0 + j) * cols + c0 + j];
+            M->re[M->diagOffsets[k] + j] = z.real();
+            M->im[M->diagOffsets[k] + j] = z.imag();
+        }
+    }
+    return M;
+}
+
+static std::shared_ptr<HostDia> from_numpy_shape(const cplx *v, const std::vector<unsigned> &shape) {
+    if (shape.size() > 2) {  // higher-order tensor: the closest factor pair as the matrix shape (conversions.cpp:193-222)
+        unsigned long long n = 1;
+        for (unsigned s : shape) n *= s;
+        unsigned x = 1, y = (unsigned)n;
+        for (unsigned i = (unsigned)std::sqrt((double)n); i > 0; --i)
+            if (n % i == 0) { x = i; y = (unsigned)(n / i); break; }
+        return diagonals_from_dense({x, y}, shape, v);
+    }
+    if (shape.size() != 2) throw std::runtime_error("from_numpy: needs a matrix (2 or more dimensions)");
+    return diagonals_from_dense({shape[0], shape[1]}, shape, v);
+}
+
+static py::array_t<cplx> to_numpy(const HostDia &A) {
+    py::array_t<cplx> out({(py::ssize_t)A.matrix_shape[0], (py::ssize_t)A.matrix_shape[1]});
+    cplx *o = out.mutable_data();
+    std::fill(o, o + (size_t)A.matrix_shape[0] * A.matrix_shape[1], cplx(0, 0));
+    for (size_t k = 0; k < A.dIndices.size(); ++k) {
+        const int d = A.dIndices[k];
+        for (unsigned j = 0; j < A.len(k); ++j) {
+            const size_t r = d >= 0 ? j : j - d, c = d >= 0 ? j + d : j;
+            o[r * A.matrix_shape[1] + c] = cplx(A.re[A.diagOffsets[k] + j], A.im[A.diagOffsets[k] + j]);
+        }
+    }
+    return out;
+}
+
+static std::shared_ptr<HostVector> vector_from(py::array_t<cplx, py::array::c_style | py::array::forcecast> a) {
+    auto v = std::make_shared<HostVector>((unsigned)a.size());
+    const cplx *p = a.data();
+    for (unsigned i = 0; i < v->numRows; ++i) { v->re[i] = p[i].real(); v->im[i] = p[i].imag(); }
+    return v;
+}
+
+static py::array_t<cplx> vector_to(const HostVector &v) {
+    py::array_t<cplx> out(v.numRows);
+    cplx *o = out.mutable_data();
+    for (unsigned i = 0; i < v.numRows; ++i) o[i] = cplx(v.re[i], v.im[i]);
+    return out;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Operations
+// ---------------------------------------------------------------------------------------------------------------------
+static std::shared_ptr<HostVector> spmv_dev(const DeviceMatrix &A, const HostVector &x) {
+    auto ms = A.matrix_shape();
+    if (x.numRows != ms[1]) throw std::runtime_error("spmv_cuda: dimension mismatch (x length vs A.cols)");
+    auto y = std::make_shared<HostVector>(ms[0]);
+    check(dq_spmv_host(A.h.get(), x.re.data(), x.im.data(), y->re.data(), y->im.data()));
+    return y;
+}
+
+static void spmv_into(const HostDia &A, const HostVector &x, HostVector &y) {
+    if (x.numRows != A.matrix_shape[1]) throw std::runtime_error("spmv_cuda: dimension mismatch (x length vs A.cols)");
+    if (y.numRows != A.matrix_shape[0]) throw std::runtime_error("spmv_cuda: dimension mismatch (y length vs A.rows)");
+    auto D = upload(A);  // a host matrix is uploaded for this call, as in the reference (src/spMV_gpu.cu:187-192)
+    check(dq_spmv_host(D->h.get(), x.re.data(), x.im.data(), y.re.data(), y.im.data()));
+}
+
+static void spmv_ntimes_into(const HostDia &A, const HostVector &x, HostVector &y, int n) {
+    if (n < 0) throw std::runtime_error("spmv_ntimes_cuda: num_times must be >= 0");
+    if (x.numRows != A.matrix_shape[1])
+        throw std::runtime_error("spmv_ntimes_cuda: dimension mismatch (x length vs A.cols)");
+    if (y.numRows != A.matrix_shape[0])
+        throw std::runtime_error("spmv_ntimes_cuda: dimension mismatch (y length vs A.rows)");
+    auto D = upload(A);
+    check(dq_spmv_ntimes_host(D->h.get(), x.re.data(), x.im.data(), y.re.data(), y.im.data(), n));
+}
+
+static std::shared_ptr<DeviceMatrix> product_dev(const DeviceMatrix &A, const DeviceMatrix &B) {
+    dq_devmat *p = nullptr;
+    check(dq_product_device(A.h.get(), B.h.get(), &p));
+    return wrap(p);
+}
+
+static std::shared_ptr<HostDia> product_host(const HostDia &A, const HostDia &B) {
+    if (A.matrix_shape[1] != B.matrix_shape[0])
+        throw std::runtime_error("diaQMatrix product_cuda: dimension mismatch.");
+    auto dA = upload(A), dB = upload(B);
+    dq_devmat *p = nullptr, *q = nullptr;
+    check(dq_product_device(dA->h.get(), dB->h.get(), &p));
+    DeviceMatrix P(p);
+    check(dq_prune(P.h.get(), &q));  // product() prunes all-zero diagonals (src/spGEMM.cpp:516-534)
+    return download(DeviceMatrix(q));
+}
+
+static std::shared_ptr<HostDia> pow_host(const HostDia &A, long long n) {
+    if (n < 0) throw std::runtime_error("pow_cuda: negative exponent not supported");
+    auto D = upload(A);
+    dq_devmat *p = nullptr;
+    check(dq_pow(D->h.get(), n, &p));
+    return download(DeviceMatrix(p));
+}
+
+static std::shared_ptr<HostDia> expm_host(const HostDia &A, int max_it, double cutoff, double tol) {
+    auto D = upload(A);
+    dq_devmat *p = nullptr;
+    check(dq_expm_taylor_scaled(D->h.get(), max_it, cutoff, tol, &p));
+    return download(DeviceMatrix(p));
+}
+
+static std::shared_ptr<HostDia> expm_negative_host(const HostDia &A, double t, int max_it, double cutoff,
+                                                   double tol) {
+    auto D = upload(A);
+    dq_devmat *p = nullptr;
+    check(dq_expm_negative_taylor_scaled(D->h.get(), t, max_it, cutoff, tol, &p, nullptr, nullptr));
+    return download(DeviceMatrix(p));
+}
+
+static cplx trace_dev(const DeviceMatrix &D) {
+    double r = 0, i = 0;
+    check(dq_trace(D.h.get(), &r, &i));
+    return cplx(r, i);
+}
+
+static cplx trace_host(const HostDia &A) {
+    const int k = A.slot(0);
+    if (k < 0) return cplx(0, 0);
+    HostDia main({A.matrix_shape[0], A.matrix_shape[1]}, {0}, A.shape);  // only the main diagonal goes up
+    std::copy(A.re.begin() + A.diagOffsets[k], A.re.begin() + A.diagOffsets[k] + A.len(k), main.re.begin());
+    std::copy(A.im.begin() + A.diagOffsets[k], A.im.begin() + A.diagOffsets[k] + A.len(k), main.im.begin());
+    return trace_dev(*upload(main));
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+PYBIND11_MODULE(diaq_gpu, m) {
+    m.doc() = "DiaQ GPU path on MI355X (gfx950): diagonal-format sparse linear algebra in HIP";
+    m.def("version", []() { return std::string(dq_version()); }, "the library version");
+
+    py::class_<HostDia, std::shared_ptr<HostDia>>(m, "diaQMatrix")
+        .def(py::init([](unsigned s) { return std::make_shared<HostDia>(std::array<unsigned, 2>{s, s},
+                                                                         std::vector<int>{}, std::vector<unsigned>{s, s}); }))
+        .def(py::init([](unsigned s, unsigned nd) {
+            return std::make_shared<HostDia>(std::array<unsigned, 2>{s, s}, symmetric_order(nd),
+                                             std::vector<unsigned>{s, s});
+        }))
+        .def(py::init([](std::array<unsigned, 2> ms) {
+            return std::make_shared<HostDia>(ms, std::vector<int>{}, std::vector<unsigned>{ms[0], ms[1]});
+        }))
+        .def(py::init([](std::array<unsigned, 2> ms, const std::vector<int> &d) {
+            return std::make_shared<HostDia>(ms, d, std::vector<unsigned>{ms[0], ms[1]});
+        }))
+        .def(py::init([](std::array<unsigned, 2> ms, const std::vector<int> &d, std::vector<unsigned> ts) {
+            return std::make_shared<HostDia>(ms, d, ts);
+        }))
+        .def_readonly("matrix_shape", &HostDia::matrix_shape)
+        .def_readonly("shape", &HostDia::shape)
+        .def_readonly("dIndices", &HostDia::dIndices)
+        .def_readonly("diagOffsets", &HostDia::diagOffsets)
+        .def_readonly("totalLength", &HostDia::totalLength)
+        .def("num_diags", [](const HostDia &A) { return (unsigned)A.dIndices.size(); })
+        .def("get_diagonals", [](const HostDia &A) { return A.dIndices; })
+        .def("copy", &HostDia::copy)
+        .def("set_at_diag", [](HostDia &A, int d, unsigned pos, double r, double i) {
+            const int k = A.slot(d);
+            if (k < 0) throw std::runtime_error("set_diag: diagonal index not present in dIndices.");
+            A.re[A.diagOffsets[k] + pos] = r;
+            A.im[A.diagOffsets[k] + pos] = i;
+        })
+        .def("getItem", [](const HostDia &A, std::array<unsigned, 2> rc) {
+            const int k = A.slot((int)rc[1] - (int)rc[0]);
+            if (k < 0) return cplx(0, 0);
+            const unsigned p = (int)rc[1] - (int)rc[0] >= 0 ? rc[0] : rc[1];
+            return cplx(A.re[A.diagOffsets[k] + p], A.im[A.diagOffsets[k] + p]);
+        })
+        .def("getMatrixPosition", [](const HostDia &A, int d, unsigned p) {
+            std::array<unsigned, 2> rc = d >= 0 ? std::array<unsigned, 2>{p, (unsigned)d + p}
+                                                : std::array<unsigned, 2>{(unsigned)(-d) + p, p};
+            if (rc[0] >= A.matrix_shape[0] || rc[1] >= A.matrix_shape[1])
+                throw std::length_error("out of matrix bounds");
+            return rc;
+        }, py::arg("diagonal_idx"), py::arg("diagonal_pos"))
+        .def("multiplyScalar", [](HostDia &A, double r, double i) {
+            for (unsigned t = 0; t < A.totalLength; ++t) {
+                const double a = A.re[t], b = A.im[t];
+                A.re[t] = a * r - b * i;
+                A.im[t] = a * i + b * r;
+            }
+        }, py::arg("real_scale_factor"), py::arg("imag_scale_factor"))
+        .def("pruneMatrix", [](HostDia &A) {
+            std::vector<int> keep;
+            for (size_t k = 0; k < A.dIndices.size(); ++k) {
+                bool nz = false;
+                for (unsigned j = 0; j < A.len(k) && !nz; ++j)
+                    nz = A.re[A.diagOffsets[k] + j] != 0.0 || A.im[A.diagOffsets[k] + j] != 0.0;
+                if (nz) keep.push_back((int)k);
+            }
+            std::vector<int> d;
+            for (int k : keep) d.push_back(A.dIndices[k]);
+            HostDia P(A.matrix_shape, d, A.shape);
+            for (size_t j = 0; j < keep.size(); ++j)
+                for (unsigned t = 0; t < P.len(j); ++t) {
+                    P.re[P.diagOffsets[j] + t] = A.re[A.diagOffsets[keep[j]] + t];
+                    P.im[P.diagOffsets[j] + t] = A.im[A.diagOffsets[keep[j]] + t];
+                }
+            A = P;
+        })
+        .def("get_diagonal", [](const HostDia &A, int d) {
+            const int k = A.slot(d);
+            if (k < 0) throw std::runtime_error("diagonal index not present in dIndices.");
+            py::array_t<cplx> out(A.len(k));
+            cplx *o = out.mutable_data();
+            for (unsigned j = 0; j < A.len(k); ++j) o[j] = cplx(A.re[A.diagOffsets[k] + j], A.im[A.diagOffsets[k] + j]);
+            return out;
+        }, "values of diagonal d (position order) as a complex128 array")
+        .def_static("from_diagonals", [](std::array<unsigned, 2> ms, const std::vector<int> &offs,
+                                         const std::vector<py::array_t<cplx, py::array::c_style | py::array::forcecast>> &vals) {
+            if (offs.size() != vals.size()) throw std::runtime_error("from_diagonals: one value array per offset");
+            auto M = std::make_shared<HostDia>(ms, offs, std::vector<unsigned>{ms[0], ms[1]});
+            for (size_t k = 0; k < offs.size(); ++k) {
+                if ((size_t)vals[k].size() != M->len(k))
+                    throw std::runtime_error("from_diagonals: diagonal " + std::to_string(offs[k]) + " needs " +
+                                             std::to_string(M->len(k)) + " values");
+                const cplx *p = vals[k].data();
+                for (unsigned j = 0; j < M->len(k); ++j) {
+                    M->re[M->diagOffsets[k] + j] = p[j].real();
+                    M->im[M->diagOffsets[k] + j] = p[j].imag();
+                }
+            }
+            return M;
+        }, "bulk constructor from per-diagonal complex arrays (no reference counterpart)")
+        .def("print_meta", [](const HostDia &A) {
+            std::cout << "matrix_shape (" << A.matrix_shape[0] << "," << A.matrix_shape[1] << ") diagonals "
+                      << A.dIndices.size() << " totalLength " << A.totalLength << std::endl;
+        });
+
+    py::class_<HostIdentity, HostDia, std::shared_ptr<HostIdentity>>(m, "identityMatrix").def(py::init<unsigned>());
+
+    py::class_<HostVector, std::shared_ptr<HostVector>>(m, "vector")
+        .def(py::init<unsigned>())
+        .def_readonly("numRows", &HostVector::numRows)
+        .def("setValue", [](HostVector &v, unsigned i, double r, double im) { v.re.at(i) = r; v.im.at(i) = im; })
+        .def("getArray", [](const HostVector &v) {
+            std::vector<cplx> out(v.numRows);
+            for (unsigned i = 0; i < v.numRows; ++i) out[i] = cplx(v.re[i], v.im[i]);
+            return out;
+        })
+        .def("copy", [](const HostVector &v) { return std::make_shared<HostVector>(v); })
+        .def("norm", [](const HostVector &v) {
+            double s = 0;
+            for (unsigned i = 0; i < v.numRows; ++i) s += v.re[i] * v.re[i] + v.im[i] * v.im[i];
+            return std::sqrt(s);
+        });
+
+    // conversions
+    m.def("from_numpy", [](py::array_t<cplx, py::array::c_style | py::array::forcecast> vals, std::vector<unsigned> shape) {
+        if (vals.ndim() != 1) throw std::runtime_error("Input arrays must be 1-dimensional.");
+        return from_numpy_shape(vals.data(), shape);
+    });
+    m.def("from_numpy", [](py::array_t<cplx, py::array::c_style | py::array::forcecast> vals) {
+        std::vector<unsigned> shape;
+        for (py::ssize_t i = 0; i < vals.ndim(); ++i) shape.push_back((unsigned)vals.shape(i));
+        return from_numpy_shape(vals.data(), shape);
+    });
+    m.def("to_numpy", [](std::shared_ptr<HostDia> A) { return to_numpy(*A); });
+    m.def("from_numpy_vector", &vector_from, py::arg());
+    m.def("to_numpy_vector", [](std::shared_ptr<HostVector> v) { return vector_to(*v); }, py::arg());
+
+    m.def("dia_viability", [](int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
+                              int itemsize, int index_bytes) {
+        dq_dia_viability_t v;
+        check(dq_dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize, index_bytes, &v));
+        py::dict d;
+        d["viable"] = (bool)v.viable;
+        d["storage_ok"] = (bool)v.storage_ok;
+        d["diagonals_ok"] = (bool)v.diagonals_ok;
+        d["dia_bytes"] = v.dia_bytes;
+        d["csr_bytes"] = v.csr_bytes;
+        d["density"] = v.density;
+        d["diagonal_budget"] = v.diagonal_budget;
+        d["reason"] = std::string(v.reason);
+        return d;
+    }, py::arg("n_rows"), py::arg("n_cols"), py::arg("n_diags"), py::arg("nnz"), py::arg("n_right"),
+       py::arg("itemsize") = 16, py::arg("index_bytes") = 8);
+
+    // kernels on host matrices (each call uploads, as the reference's host API does)
+    m.def("product", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostDia> B) { return product_host(*A, *B); });
+    m.def("trace", [](std::shared_ptr<HostDia> A) { return trace_host(*A); });
+    m.def("spMV", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x) {
+        auto y = std::make_shared<HostVector>(A->matrix_shape[0]);
+        spmv_into(*A, *x, *y);
+        return y;
+    }, py::arg(), py::arg());
+    m.def("spMV", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x, std::shared_ptr<HostVector> y) {
+        spmv_into(*A, *x, *y);
+        return y;
+    }, py::arg(), py::arg(), py::arg());
+    m.def("spMV_ntimes", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x, int n) {
+        auto y = std::make_shared<HostVector>(A->matrix_shape[0]);
+        spmv_ntimes_into(*A, *x, *y, n);
+        return y;
+    }, py::arg(), py::arg(), py::arg());
+    m.def("spMV_ntimes", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x, std::shared_ptr<HostVector> y,
+                            int n) {
+        spmv_ntimes_into(*A, *x, *y, n);
+        return y;
+    }, py::arg(), py::arg(), py::arg(), py::arg());
+    m.def("pow", [](std::shared_ptr<HostDia> A, long long n) { return pow_host(*A, n); }, py::arg(), py::arg());
+    m.def("expm", [](std::shared_ptr<HostDia> A, int max_iterations, double scale_cutoff, double tolerance) {
+        return expm_host(*A, max_iterations, scale_cutoff, tolerance);
+    }, py::arg("A"), py::arg("max_iterations") = 100, py::arg("scale_cutoff") = 1.0, py::arg("tolerance") = 1e-12);
+    m.def("expm_negative", [](std::shared_ptr<HostDia> A, double t, int max_iterations, double scale_cutoff,
+                              double tolerance) {
+        return expm_negative_host(*A, t, max_iterations, scale_cutoff, tolerance);
+    }, py::arg("A"), py::arg("t"), py::arg("max_iterations") = 100, py::arg("scale_cutoff") = 1.0,
+       py::arg("tolerance") = 1e-12);
+
+    // device-resident matrices
+    py::class_<DeviceMatrix, std::shared_ptr<DeviceMatrix>>(m, "DeviceMatrix",
+        "A diaQMatrix in GPU memory (uploaded once). spMV(DeviceMatrix, vector) copies only the vectors; "
+        "product(DeviceMatrix, DeviceMatrix) returns a DeviceMatrix without leaving the GPU; to_host() downloads.")
+        .def(py::init([](std::shared_ptr<HostDia> A, const std::string &precision) {
+            return upload(*A, parse_precision(precision));
+        }), py::arg("A"), py::arg("precision") = "fp64")
+        .def("to_host", [](const DeviceMatrix &D) { return download(D); })
+        .def("copy", [](const DeviceMatrix &D) {
+            dq_devmat *p = nullptr;
+            check(dq_devmat_copy(D.h.get(), &p));
+            return wrap(p, D.tensor_shape);
+        })
+        .def("astype", [](const DeviceMatrix &D, const std::string &precision) {
+            dq_devmat *p = nullptr;
+            check(dq_devmat_convert(D.h.get(), parse_precision(precision), &p));
+            return wrap(p, D.tensor_shape);
+        }, py::arg("precision"), "a copy in 'fp64' or 'fp32' (no reference counterpart)")
+        .def("get_diagonal", [](const DeviceMatrix &D, int offset) {
+            auto ms = D.matrix_shape();
+            const unsigned n = diag_len(offset, ms);
+            std::vector<double> r(n), i(n);
+            check(dq_devmat_get_diagonal(D.h.get(), offset, r.data(), i.data()));
+            py::array_t<cplx> out(n);
+            cplx *o = out.mutable_data();
+            for (unsigned j = 0; j < n; ++j) o[j] = cplx(r[j], i[j]);
+            return out;
+        }, py::arg("offset"), "values of one diagonal (position order), downloaded (no reference counterpart)")
+        .def_property_readonly("shape", &DeviceMatrix::matrix_shape)
+        .def_property_readonly("num_diags", &DeviceMatrix::n_diags)
+        .def_property_readonly("diagonals", &DeviceMatrix::diagonals)
+        .def_property_readonly("precision", [](const DeviceMatrix &D) {
+            return std::string(D.precision() == DQ_F32 ? "fp32" : "fp64");
+        })
+        .def_property_readonly("stored", [](const DeviceMatrix &D) {
+            int64_t s;
+            check(dq_devmat_info(D.h.get(), nullptr, nullptr, nullptr, nullptr, &s));
+            return s;
+        })
+        .def_property_readonly("handle", [](const DeviceMatrix &D) { return reinterpret_cast<uintptr_t>(D.h.get()); },
+                               "the dq_devmat pointer for the C ABI");
+    m.def("to_device", [](std::shared_ptr<HostDia> A, const std::string &precision) {
+        return upload(*A, parse_precision(precision));
+    }, py::arg("A"), py::arg("precision") = "fp64", "Upload a diaQMatrix to the GPU (a DeviceMatrix).");
+    m.def("spMV", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<HostVector> x) { return spmv_dev(*A, *x); },
+          py::arg(), py::arg());
+    m.def("product", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<DeviceMatrix> B) {
+        return product_dev(*A, *B);
+    }, py::arg(), py::arg());
+    m.def("trace", [](std::shared_ptr<DeviceMatrix> A) { return trace_dev(*A); });
+    m.def("prune", [](std::shared_ptr<DeviceMatrix> A) {
+        dq_devmat *p = nullptr;
+        check(dq_prune(A->h.get(), &p));
+        return wrap(p);
+    }, "drop all-zero diagonals on the device (no reference counterpart for a DeviceMatrix)");
+    m.def("estimate_1norm", [](std::shared_ptr<DeviceMatrix> A) {
+        double v;
+        check(dq_estimate_1norm(A->h.get(), &v));
+        return v;
+    });
+    m.def("norm", [](std::shared_ptr<DeviceMatrix> A) {
+        double v;
+        check(dq_frobenius_norm(A->h.get(), &v));
+        return v;
+    });
+    m.def("norm_diff", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<DeviceMatrix> B) {
+        double v;
+        check(dq_frobenius_diff(A->h.get(), B->h.get(), &v));
+        return v;
+    }, "||A - B||_F on the device for matrices with the same diagonals (no reference counterpart)");
+    m.def("spmv_device", [](std::shared_ptr<DeviceMatrix> A, uintptr_t xr, uintptr_t xi, uintptr_t yr, uintptr_t yi,
+                            uintptr_t stream) {
+        check(dq_spmv_device(A->h.get(), (const void *)xr, (const void *)xi, (void *)yr, (void *)yi, stream));
+    }, py::arg("A"), py::arg("x_real"), py::arg("x_imag"), py::arg("y_real"), py::arg("y_imag"), py::arg("stream") = 0,
+       "y = A x with x, y as GPU arrays of float64 (float32 for an fp32 DeviceMatrix), real and imaginary parts "
+       "(e.g. torch data pointers); y is overwritten; asynchronous on `stream`.");
+    m.def("spmm_dia_device", [](uintptr_t offsets, uintptr_t vals, int n_diags, int64_t n_rows, int64_t n_cols,
+                                uintptr_t B, int64_t n_right, uintptr_t C, bool is_complex, uintptr_t stream) {
+        check(dq_spmm_dia_device(offsets, vals, n_diags, n_rows, n_cols, B, n_right, C, is_complex ? 1 : 0, stream));
+    }, py::arg("offsets"), py::arg("vals"), py::arg("n_diags"), py::arg("n_rows"), py::arg("n_cols"), py::arg("B"),
+       py::arg("n_right"), py::arg("C"), py::arg("is_complex"), py::arg("stream") = 0,
+       "C = A * B, A in row-aligned DIA format (offsets int64[n_diags], vals[n_diags * n_rows]); B (n_cols x n_right) "
+       "and C (n_rows x n_right) row-major; double or complex double; all pointers on the GPU.");
+    m.def("hamiltonian_simulation_cuda", [](std::shared_ptr<HostDia> H, std::shared_ptr<HostVector> psi0, double dt,
+                                            unsigned long r, int max_iterations, double scale_cutoff,
+                                            double tolerance) {
+        (void)scale_cutoff;  // accepted and ignored, as in the reference (src/spHamSim_gpu_all_fused.cu:293-294)
+        (void)tolerance;
+        if (H->matrix_shape[0] != H->matrix_shape[1])
+            throw std::runtime_error("hamiltonian_simulation_cuda: H must be square");
+        if (psi0->numRows != H->matrix_shape[1])
+            throw std::runtime_error("hamiltonian_simulation_cuda: psi0 dimension mismatch");
+        auto out = std::make_shared<HostVector>(psi0->numRows);
+        if (psi0->numRows == 0) return out;
+        auto D = upload(*H);
+        check(dq_hamsim_host(D->h.get(), psi0->re.data(), psi0->im.data(), dt, r, max_iterations, out->re.data(),
+                             out->im.data()));
+        return out;
+    }, py::arg("H"), py::arg("psi0"), py::arg("dt"), py::arg("r"), py::arg("max_iterations") = 100,
+       py::arg("scale_cutoff") = 1.0, py::arg("tolerance") = 1e-12);
+    m.def("hamsim_device", [](std::shared_ptr<DeviceMatrix> H, uintptr_t pr, uintptr_t pi, double t, uint64_t r, int m,
+                              uintptr_t stream) {
+        check(dq_hamsim_device(H->h.get(), (void *)pr, (void *)pi, t, r, m, stream));
+    }, py::arg("H"), py::arg("psi_real"), py::arg("psi_imag"), py::arg("t"), py::arg("r"), py::arg("m"),
+       py::arg("stream") = 0, "the fused evolution in place on device vectors (no reference counterpart)");
+
+    m.def("install_info", []() {
+        py::dict info;
+        info["VEC_WIDTH"] = (int)VEC_WIDTH;
+        info["GPU_ARCH"] = "gfx950";
+        info["HIP_ENABLED"] = true;
+        info["CUDA_ENABLED"] = false;
+        info["valType"] = "d";
+        info["library"] = std::string(dq_version());
+        return info;
+    }, "build configuration as a dict");
+}

@@ -1,0 +1,126 @@
+// DIA x dense on device pointers (torch / cupy data pointers): C = A B with A in row-aligned DIA form.
+//
+// Reference: spmm_dia_kernel / launch_rows / launch / spmm_dia_device (src/spMM_device_gpu.cu:44-127) and the
+// viability rule dia_viability (src/dia_viability.hpp:42-63).
+//
+// Layout (as the reference): vals[k * n_rows + r] = A[r, r + offsets[k]]; B is n_cols x n_right and C n_rows x
+// n_right, both row-major; double or interleaved complex double.
+//
+// Mapping on gfx950: lane e owns C's flat element e = r * n_right + j. Its B operand for diagonal k is
+// B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run of B and
+// writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles rows,
+// which coalesces only for wide B). The diagonal values vals[k * n_rows + r] are shared by the n_right lanes of a row.
+// Offsets are staged in LDS (up to 2048 of them). Each element sums its diagonals in order, as the reference does.
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "dq_common.hpp"
+
+namespace dq {
+
+constexpr int SPMM_LDS_OFFSETS = 2048;
+
+struct cplx {
+    double x, y;
+};
+
+__device__ inline double fma_val(double acc, double a, double b) { return fma(a, b, acc); }
+__device__ inline cplx fma_val(cplx acc, cplx a, cplx b) {
+    return cplx{acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ inline T zero_val();
+template <> __device__ inline double zero_val<double>() { return 0.0; }
+template <> __device__ inline cplx zero_val<cplx>() { return cplx{0.0, 0.0}; }
+
+template <typename T, typename I>
+__global__ __launch_bounds__(256) void spmm_dia_kernel(I n_rows, I n_cols, I n_right, int n_diags,
+                                                       const int64_t *__restrict__ offsets, const T *__restrict__ vals,
+                                                       const T *__restrict__ B, T *__restrict__ C) {
+    __shared__ int64_t s_off[SPMM_LDS_OFFSETS];
+    const bool staged = n_diags <= SPMM_LDS_OFFSETS;
+    if (staged) {
+        for (int k = threadIdx.x; k < n_diags; k += blockDim.x) s_off[k] = offsets[k];
+        __syncthreads();
+    }
+    const I total = n_rows * n_right;
+    for (I e = static_cast<I>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<I>(gridDim.x) * blockDim.x) {
+        const I r = e / n_right;
+        T acc = zero_val<T>();
+        for (int k = 0; k < n_diags; ++k) {
+            const int64_t o = staged ? s_off[k] : offsets[k];
+            const int64_t c = static_cast<int64_t>(r) + o;
+            if (c >= 0 && c < static_cast<int64_t>(n_cols))
+                acc = fma_val(acc, vals[static_cast<int64_t>(k) * n_rows + r], B[static_cast<int64_t>(e) + o * n_right]);
+        }
+        C[e] = acc;
+    }
+}
+
+template <typename T>
+static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off, const T *vals,
+                   const T *B, T *C, hipStream_t s) {
+    const int64_t total = n_rows * n_right;
+    if (total == 0) return;
+    const unsigned grid = blocks_for(total, 256, 1 << 16);
+    if (total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31))
+        hipLaunchKernelGGL((spmm_dia_kernel<T, uint32_t>), dim3(grid), dim3(256), 0, s, (uint32_t)n_rows,
+                           (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C);
+    else
+        hipLaunchKernelGGL((spmm_dia_kernel<T, int64_t>), dim3(grid), dim3(256), 0, s, n_rows, n_cols, n_right,
+                           n_diags, off, vals, B, C);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) throw HipError(std::string("spmm_dia_device: ") + hipGetErrorString(err));
+}
+
+}  // namespace dq
+
+using namespace dq;
+
+extern "C" int dq_spmm_dia_device(uintptr_t offsets, uintptr_t vals, int n_diags, int64_t n_rows, int64_t n_cols,
+                                  uintptr_t B, int64_t n_right, uintptr_t C, int is_complex, uintptr_t stream) {
+    return guard([&] {
+        if (n_diags < 0 || n_rows < 0 || n_cols < 0 || n_right < 0) throw ArgError("spmm_dia_device: negative size");
+        if (n_rows * n_right == 0) return;
+        if (!C || (n_diags > 0 && (!offsets || !vals)) || (n_cols > 0 && !B))
+            throw ArgError("spmm_dia_device: NULL pointer");
+        const auto *off = reinterpret_cast<const int64_t *>(offsets);
+        if (is_complex)
+            launch<cplx>(n_rows, n_cols, n_right, n_diags, off, reinterpret_cast<const cplx *>(vals),
+                         reinterpret_cast<const cplx *>(B), reinterpret_cast<cplx *>(C), as_stream(stream));
+        else
+            launch<double>(n_rows, n_cols, n_right, n_diags, off, reinterpret_cast<const double *>(vals),
+                           reinterpret_cast<const double *>(B), reinterpret_cast<double *>(C), as_stream(stream));
+    });
+}
+
+// Storage and diagonal budgets (src/dia_viability.hpp:36-63): DIA bytes <= 0.9 x CSR bytes, and
+// n_diags <= max(16, n_rows / 32), capped at 8 when n_right >= 32.
+extern "C" int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
+                                int itemsize, int index_bytes, dq_dia_viability_t *out) {
+    return guard([&] {
+        if (!out) throw ArgError("out is NULL");
+        (void)n_cols;
+        dq_dia_viability_t v;
+        std::memset(&v, 0, sizeof(v));
+        const double stored = static_cast<double>(n_diags) * static_cast<double>(n_rows);
+        v.dia_bytes = stored * itemsize + static_cast<double>(n_diags) * 8.0;
+        v.csr_bytes = static_cast<double>(nnz) * (itemsize + index_bytes) + static_cast<double>(n_rows + 1) * index_bytes;
+        v.density = stored > 0 ? static_cast<double>(nnz) / stored : 0.0;
+        v.storage_ok = n_diags > 0 && v.dia_bytes <= 0.9 * v.csr_bytes;
+        v.diagonal_budget = std::max<int64_t>(16, n_rows / 32);
+        if (n_right >= 32) v.diagonal_budget = std::min<int64_t>(v.diagonal_budget, 8);
+        v.diagonals_ok = n_diags <= v.diagonal_budget;
+        v.viable = v.storage_ok && v.diagonals_ok;
+        std::string reason;
+        if (!v.storage_ok)
+            reason = "storage: DIA would take more than 90% of CSR's bytes (diagonals too sparsely filled)";
+        else if (!v.diagonals_ok)
+            reason = "diagonals: " + std::to_string(n_diags) + " over the budget of " + std::to_string(v.diagonal_budget);
+        else
+            reason = "viable";
+        std::strncpy(v.reason, reason.c_str(), sizeof(v.reason) - 1);
+        *out = v;
+    });
+}

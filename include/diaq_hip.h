@@ -1,0 +1,200 @@
+/*
+ * diaq_hip.h -- C ABI of the MI355X (gfx950) DiaQ GPU library, libdiaq_hip.so.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types cross this boundary. Every entry point returns
+ * DQ_OK (0) or an error code, and dq_last_error() gives the message of the calling thread's last failure (the texts
+ * follow the reference's exception messages, so a binding can rethrow them unchanged).
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the reference checkout). Pointers
+ * named *_dev are device pointers the caller owns (borrowed for the call); streams travel as uintptr_t
+ * (hipStream_t; 0 is the null stream). Entry points without a stream argument run on the null stream and return
+ * after their results are complete.
+ */
+#ifndef DIAQ_HIP_H
+#define DIAQ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DQ_OK = 0,
+    DQ_ERR_ARG = 1,    /* bad argument: shape mismatch, bad size, null pointer ... */
+    DQ_ERR_HIP = 2,    /* a HIP runtime call failed */
+    DQ_ERR_ALLOC = 3,  /* device or host allocation failed */
+    DQ_ERR_STATE = 4   /* anything else */
+};
+
+enum { DQ_F64 = 0, DQ_F32 = 1 }; /* value precision of a device matrix */
+
+/* Opaque device-resident DIA matrix: deviceDiaQMatrix (src/deviceMatrix_gpu.hpp:37-68). */
+typedef struct dq_devmat dq_devmat;
+
+/* Message of this thread's last failed call ("" if none). */
+const char *dq_last_error(void);
+
+/* Library version string. Replaces version() (include/pythonWrappers.cpp:15-17). */
+const char *dq_version(void);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Device matrices
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* Upload a host DIA matrix: n_diags diagonals with offsets[k]; diagonal k's values start at element starts[k] of the
+ * host arrays re / im (so the reference's SIMD-padded host layout and a packed one both work). precision DQ_F64
+ * keeps the values; DQ_F32 rounds them to float. Replaces deviceDiaQMatrix(const shared_ptr<diaQMatrix>&)
+ * (src/deviceMatrix_gpu.hpp:46, src/deviceMatrix_gpu.cu:62-76). */
+int dq_devmat_upload(uint32_t rows, uint32_t cols, int32_t n_diags, const int32_t *offsets, const int64_t *starts,
+                     const double *re, const double *im, int precision, dq_devmat **out);
+
+/* A matrix with these diagonals, all values zero. Replaces deviceDiaQMatrix(shape, diagonals)
+ * (src/deviceMatrix_gpu.hpp:48, src/deviceMatrix_gpu.cu:56-60). */
+int dq_devmat_zeros(uint32_t rows, uint32_t cols, int32_t n_diags, const int32_t *offsets, int precision,
+                    dq_devmat **out);
+
+/* Fill all values from packed device arrays (diagonal k's len_k values back to back, in slot order) of the
+ * matrix precision, on `stream`. No reference counterpart: the bulk constructor for device-generated matrices. */
+int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, const void *im_dev, uintptr_t stream);
+
+/* Device-to-device copy. Replaces deviceDiaQMatrix::copy (src/deviceMatrix_gpu.hpp:53, .cu:87-95). */
+int dq_devmat_copy(const dq_devmat *A, dq_devmat **out);
+
+/* A copy in another precision (FP64 <-> FP32). No reference counterpart (precision form). */
+int dq_devmat_convert(const dq_devmat *A, int precision, dq_devmat **out);
+
+/* Download into host arrays laid out by starts[] (as dq_devmat_upload); values are widened to double.
+ * Replaces deviceDiaQMatrix::to_host (src/deviceMatrix_gpu.hpp:55, .cu:97-113). */
+int dq_devmat_download(const dq_devmat *A, const int64_t *starts, double *re, double *im);
+
+/* Download one diagonal (by offset) into host arrays of its length, widened to double. No reference counterpart (the
+ * reference downloads whole matrices only). */
+int dq_devmat_get_diagonal(const dq_devmat *A, int32_t offset, double *re, double *im);
+
+/* Free the matrix (NULL is ignored). Replaces ~deviceDiaQMatrix (src/deviceMatrix_gpu.cu:78-85). */
+void dq_devmat_destroy(dq_devmat *A);
+
+/* Shape, number of diagonals, precision, stored elements (sum of diagonal lengths). Replaces the matrix_shape /
+ * num_diags members (src/deviceMatrix_gpu.hpp:39, :60). */
+int dq_devmat_info(const dq_devmat *A, uint32_t *rows, uint32_t *cols, int32_t *n_diags, int *precision,
+                   int64_t *stored);
+
+/* The diagonal offsets in slot order (n_diags entries). Replaces the dIndices member (src/deviceMatrix_gpu.hpp:41). */
+int dq_devmat_offsets(const dq_devmat *A, int32_t *out);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * SpMV
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* y = A x on the device, y overwritten, asynchronous on `stream`. x / y hold cols / rows values of the matrix
+ * precision (double for DQ_F64, float for DQ_F32) as split real and imaginary arrays.
+ * Replaces spmv_device (src/deviceMatrix_gpu.hpp:71-72, src/spMV_gpu.cu:142-168). */
+int dq_spmv_device(const dq_devmat *A, const void *xr_dev, const void *xi_dev, void *yr_dev, void *yi_dev,
+                   uintptr_t stream);
+
+/* y = A x for host vectors (copied in and out). Replaces spmv_host_vectors / spmv_cuda(DeviceMatrix, x)
+ * (src/spMV_gpu.cu:171-183, :200-204). */
+int dq_spmv_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi);
+
+/* y = A^n x for host vectors, by n SpMVs on the device. Replaces spMV_ntimes_cuda (src/spMV_ntimes_gpu.cu:61-263). */
+int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi, int n);
+
+/* y = A x with the column range split into n_bands passes (n_bands = 0 picks the library's choice for the size);
+ * same result as dq_spmv_device up to summation order. No reference counterpart (tuning entry point). */
+int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi_dev, void *yr_dev, void *yi_dev,
+                          int n_bands, uintptr_t stream);
+
+/* Process-wide tuning knobs for the autotuner and benchmarks: "spmv_rows_per_lane" (0 = automatic, or 1, 2, 4, 8)
+ * and "spmv_bands" (0 = automatic, or the number of column bands). No reference counterpart. */
+int dq_set_tuning(const char *key, int value);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * SpGEMM, prune, norms
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* Structural dry-run of C = A B: the ascending result offsets and, per result diagonal, its (slot in A, slot in B)
+ * pairs in A-slot order. Call with res_offsets = NULL to get the counts, then with arrays of those sizes.
+ * Replaces the host pairing in product_device (src/spGEMM_gpu.cu:90-125). */
+int dq_product_dryrun(uint32_t rows, uint32_t cols, int32_t n_a, const int32_t *offs_a, int32_t n_b,
+                      const int32_t *offs_b, int32_t *n_res, int64_t *n_pairs, int32_t *res_offsets,
+                      int64_t *res_pair_start, int32_t *pair_slot_a, int32_t *pair_slot_b);
+
+/* C = A B on the device, without pruning. Replaces product_device (src/deviceMatrix_gpu.hpp:78,
+ * src/spGEMM_gpu.cu:84-151). */
+int dq_product_device(const dq_devmat *A, const dq_devmat *B, dq_devmat **out);
+
+/* Drop the diagonals whose values are all exactly zero. Replaces diaQMatrix::pruneMatrix on the product's result
+ * (src/diaQMatrix.hpp:1543-1604, src/spGEMM.cpp:516-534). */
+int dq_prune(const dq_devmat *A, dq_devmat **out);
+
+/* max over columns of sum(|re| + |im|). Replaces diaQMatrix::estimate_1norm (src/diaQMatrix.hpp:1287-1358). */
+int dq_estimate_1norm(const dq_devmat *A, double *out);
+
+/* sqrt(sum re^2 + im^2). Replaces diaQMatrix::norm (src/diaQMatrix.hpp:1255-1284). */
+int dq_frobenius_norm(const dq_devmat *A, double *out);
+
+/* ||A - B||_F for two matrices with the same shape and diagonals (precisions may differ). No reference counterpart:
+ * the error report of the precision form. */
+int dq_frobenius_diff(const dq_devmat *A, const dq_devmat *B, double *out);
+
+/* Sum of the main diagonal. Replaces trace (src/matrixKernels.cpp:9-26). */
+int dq_trace(const dq_devmat *A, double *re, double *im);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * pow and SpME
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* A^n (n >= 0). Replaces pow_cuda (src/spPow_gpu.cu:21-56). */
+int dq_pow(const dq_devmat *A, int64_t n, dq_devmat **out);
+
+/* exp(A) by Taylor series with scaling and squaring. Replaces expm_taylor_scaled_cuda (src/spME_gpu.cu:10-78). */
+int dq_expm_taylor_scaled(const dq_devmat *A, int max_iterations, double scale_cutoff, double tolerance,
+                          dq_devmat **out);
+
+/* exp(-i t A) by Taylor series with scaling and squaring. Replaces expm_negative_taylor_scaled_cuda
+ * (src/spME_gpu.cu:229-294). terms_out (may be NULL) receives the number of Taylor terms summed and s_out (may be
+ * NULL) the number of squarings. */
+int dq_expm_negative_taylor_scaled(const dq_devmat *A, double t, int max_iterations, double scale_cutoff,
+                                   double tolerance, dq_devmat **out, int *terms_out, int *s_out);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Fused time evolution
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* psi(t) = prod_r exp(-i H t/r) psi0 with a fixed-m Taylor action per step (m = max_iterations); host vectors in and
+ * out. H must be square FP64. Replaces hamiltonian_simulation_cuda (src/spHamSim_gpu_all_fused.cu:287-376). */
+int dq_hamsim_host(const dq_devmat *H, const double *psi_r, const double *psi_i, double t, uint64_t r,
+                   int max_iterations, double *out_r, double *out_i);
+
+/* The same evolution in place on device vectors of H's precision, asynchronous on `stream`. No reference
+ * counterpart (the resident form of the fused evolution). */
+int dq_hamsim_device(const dq_devmat *H, void *psi_r_dev, void *psi_i_dev, double t, uint64_t r, int m,
+                     uintptr_t stream);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * DIA x dense on device pointers
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* C = A B, A row-aligned DIA (offsets int64[n_diags], vals[n_diags * n_rows]: vals[k*n_rows + r] = A[r, r+off_k]),
+ * B n_cols x n_right and C n_rows x n_right row-major, double or interleaved complex double; asynchronous on
+ * `stream`. Replaces spmm_dia_device (src/spMM_device_gpu.hpp:19-20, src/spMM_device_gpu.cu:116-127). */
+int dq_spmm_dia_device(uintptr_t offsets, uintptr_t vals, int n_diags, int64_t n_rows, int64_t n_cols, uintptr_t B,
+                       int64_t n_right, uintptr_t C, int is_complex, uintptr_t stream);
+
+/* Storage and diagonal budgets for DIA x dense. Replaces dia_viability (src/dia_viability.hpp:42-63). */
+typedef struct {
+    int viable, storage_ok, diagonals_ok;
+    double dia_bytes, csr_bytes, density;
+    int64_t diagonal_budget;
+    char reason[160];
+} dq_dia_viability_t;
+
+int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right, int itemsize,
+                     int index_bytes, dq_dia_viability_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIAQ_HIP_H */

@@ -377,6 +377,39 @@ def expm(A: Dia, max_iterations: int = 100, scale_cutoff: float = 1.0, tolerance
     return pow_gpu(U, 1 << s)
 
 
+def _offsets_union(a, b):
+    return sorted(set(a) | set(b))
+
+
+def expm_negative_dense(H: np.ndarray, offsets, t: float, max_iterations: int = 100, scale_cutoff: float = 1.0,
+                        tolerance: float = 1e-12):
+    """expm_negative with dense values: the same steps as ``expm_negative`` (src/spME_gpu.cu:229-294, :156-222,
+    src/spPow_gpu.cu:21-56) with the stored-diagonal sets carried alongside through the structural dry-run, for sizes
+    where DIA products in numpy are too slow. ``H`` is dense, ``offsets`` its stored diagonals (slot order).
+    Returns (U dense, result offsets, terms summed, s). Matrices with more than one diagonal only."""
+    n = H.shape[0]
+    D = np.abs(H.real) + np.abs(H.imag)
+    s = scaling_exponent(float(D.sum(axis=0).max()), t, scale_cutoff)
+    ts = math.ldexp(t, -s)
+    tol = max(1e-15, tolerance * math.ldexp(1.0, -s))
+    X = 1j * (-1.0 * ts) * H
+    result = np.eye(n, dtype=np.complex128)
+    term = np.eye(n, dtype=np.complex128)
+    r_offs, t_offs = [0], [0]
+    i = 0
+    for i in range(1, max_iterations + 1):
+        term = (term @ X) / i
+        t_offs, _ = product_dryrun(t_offs, offsets, n, n)
+        result = result + term
+        r_offs = _offsets_union(r_offs, t_offs)
+        if np.linalg.norm(term) < tol:
+            break
+    for _ in range(s):
+        result = result @ result
+        r_offs, _ = product_dryrun(r_offs, r_offs, n, n)
+    return result, r_offs, min(i, max_iterations), s
+
+
 # ----------------------------------------------------------------------------------------------------------------------
 # Fused time evolution
 # ----------------------------------------------------------------------------------------------------------------------

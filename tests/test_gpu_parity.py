@@ -1,0 +1,421 @@
+"""GPU parity tests: the HIP path (through the diaq_gpu drop-in and the C ABI) against the oracle, the reference's
+fixtures and dense numpy/scipy. Integer results (offset sets) bit-exact; FP64 values within 1e-12 relative
+(max-abs error over max-abs value, the reference's own GPU test metric, python_tests/test_device_matrix.py:26)."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as O  # noqa: E402
+from diaq_amd import hamiltonians as Hm  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def d():
+    import diaq_amd  # noqa: F401
+    import diaq_gpu
+    return diaq_gpu
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - b)) / (den if den > 0 else 1.0))
+
+
+def banded(n, offsets, rng, cols=None):
+    cols = n if cols is None else cols
+    A = np.zeros((n, cols), complex)
+    for off in offsets:
+        k = np.arange(max(0, -off), min(n, cols - off))
+        A[k, k + off] = rng.standard_normal(len(k)) + 1j * rng.standard_normal(len(k))
+    return A
+
+
+def dia_of(d, rows, cols, offs, vals):
+    return d.diaQMatrix.from_diagonals([rows, cols], list(offs), list(vals))
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# The reference's DeviceMatrix script (python_tests/test_device_matrix.py:20-51), same inputs and bound
+def test_reference_device_matrix_script(d):
+    rng = np.random.default_rng(0)
+    n = 2048
+    A = banded(n, (-3, -1, 0, 2, 5), rng)
+    B = banded(n, (-7, -2, 0, 3, 11), rng)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dA, dB = d.to_device(d.from_numpy(A)), d.DeviceMatrix(d.from_numpy(B))
+    worst = 0.0
+    worst = max(worst, rel(d.to_numpy(dA.to_host()), A))
+    worst = max(worst, rel(d.to_numpy_vector(d.spMV(dA, d.from_numpy_vector(x))), A @ x))
+    worst = max(worst, rel(d.to_numpy(d.product(dA, dB).to_host()), A @ B))
+    worst = max(worst, rel(d.to_numpy(d.product(d.product(dA, dB), dA).to_host()), A @ B @ A))
+    worst = max(worst, rel(d.to_numpy(d.product(d.from_numpy(A), d.from_numpy(B))), A @ B))
+    single = banded(n, (4,), rng)
+    worst = max(worst, rel(d.to_numpy_vector(d.spMV(d.to_device(d.from_numpy(single)), d.from_numpy_vector(x))),
+                           single @ x))
+    small = banded(64, (-1, 0, 1), rng) / 4
+    worst = max(worst, rel(d.to_numpy(d.pow(d.from_numpy(small), 7)), np.linalg.matrix_power(small, 7)))
+    worst = max(worst, rel(d.to_numpy(d.pow(d.from_numpy(small), 8)), np.linalg.matrix_power(small, 8)))
+    assert worst < TOL, worst
+    import torch
+    xr = torch.as_tensor(x.real, device="cuda").contiguous()
+    xi = torch.as_tensor(x.imag, device="cuda").contiguous()
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    d.spmv_device(dA, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(),
+                  torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert rel((yr + 1j * yi).cpu().numpy(), A @ x) < TOL
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# SpMV
+@pytest.mark.parametrize("shape,offs", [
+    ((4096, 4096), list(range(-4, 5))),                 # python_tests/check_gpu_vs_numpy.py:7-10
+    ((3000, 5000), [-2999, -7, 0, 1, 4999]),            # extreme offsets, wide
+    ((5000, 3000), [-4999, -100, 0, 2999]),             # tall
+    ((777, 777), [13]),                                 # one off-diagonal
+    ((777, 777), [-776]),                               # one corner element
+    ((70000, 70000), [-65536, -1, 0, 3, 40000]),        # multi-tile, unsorted-free
+    ((1000, 1000), [300, -1, 0, 1, -300, 17]),          # unsorted slots
+])
+def test_spmv_vs_oracle(d, shape, offs):
+    rng = np.random.default_rng(1)
+    A = banded(shape[0], offs, rng, cols=shape[1]) if shape[0] * shape[1] <= 25_000_000 else None
+    if A is None:
+        vals = [rng.standard_normal(O.diagonal_length(o, *shape)) + 1j * rng.standard_normal(O.diagonal_length(o, *shape))
+                for o in offs]
+        M = O.Dia(shape[0], shape[1], list(offs), vals)
+        H = dia_of(d, shape[0], shape[1], offs, vals)
+    else:
+        M = O.Dia(shape[0], shape[1], list(offs),
+                  [A[O.position_to_rc(o, np.arange(O.diagonal_length(o, *shape)))] for o in offs])
+        H = dia_of(d, shape[0], shape[1], offs, M.vals)
+    x = rng.standard_normal(shape[1]) + 1j * rng.standard_normal(shape[1])
+    ref = O.spmv(M, x)
+    got = d.to_numpy_vector(d.spMV(d.to_device(H), d.from_numpy_vector(x)))
+    assert rel(got, ref) < TOL
+    got_host = d.to_numpy_vector(d.spMV(H, d.from_numpy_vector(x)))  # host API: uploads per call
+    assert rel(got_host, ref) < TOL
+
+
+def test_spmv_known_answer_and_edge_cases(d):
+    g = json.load(open(os.path.join(GOLD, "spmv_known.json")))
+    A = (np.array(g["A"]["re"]) + 1j * np.array(g["A"]["im"])).reshape(g["A"]["shape"])
+    x = np.array(g["x"]["re"]) + 1j * np.array(g["x"]["im"])
+    y = d.to_numpy_vector(d.spMV(d.from_numpy(A), d.from_numpy_vector(x)))
+    printed = np.array(g["y_printed"]["re"]) + 1j * np.array(g["y_printed"]["im"])
+    np.testing.assert_allclose(y, printed, atol=2e-6)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-14, atol=1e-15)
+    # no diagonals: y = 0; identity: y = x
+    Z = d.diaQMatrix([5, 5])
+    np.testing.assert_array_equal(d.to_numpy_vector(d.spMV(d.to_device(Z), d.from_numpy_vector(np.ones(5) + 1j))),
+                                  np.zeros(5))
+    xi = np.arange(9) + 2j
+    np.testing.assert_array_equal(d.to_numpy_vector(d.spMV(d.identityMatrix(9), d.from_numpy_vector(xi))), xi)
+
+
+def test_spmv_ntimes(d):
+    rng = np.random.default_rng(2)
+    A = banded(300, (-2, 0, 1, 7), rng) / 3
+    x = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    for n in (0, 1, 2, 5):
+        got = d.to_numpy_vector(d.spMV_ntimes(d.from_numpy(A), d.from_numpy_vector(x), n))
+        assert rel(got, np.linalg.matrix_power(A, n) @ x) < TOL
+    R = d.from_numpy(banded(200, (0, 5), rng, cols=300))
+    with pytest.raises(RuntimeError, match="square"):
+        d.spMV_ntimes(R, d.from_numpy_vector(x), 2)
+    got = d.to_numpy_vector(d.spMV_ntimes(R, d.from_numpy_vector(x), 1))
+    assert rel(got, d.to_numpy(R) @ x) < TOL
+
+
+def test_spmv_device_capi_streams_and_fp32(d):
+    import torch
+    from diaq_amd import capi
+    rng = np.random.default_rng(3)
+    N = 1 << 16
+    offs = Hm.random_offsets(N, 24, seed=3)
+    vals = [rng.standard_normal(O.diagonal_length(o, N, N)) + 1j * rng.standard_normal(O.diagonal_length(o, N, N))
+            for o in offs]
+    M = O.Dia(N, N, offs, vals)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    ref = O.spmv(M, x)
+    H = dia_of(d, N, N, offs, vals)
+    for prec, tol, dt in (("fp64", TOL, torch.float64), ("fp32", 2e-6, torch.float32)):
+        D = d.to_device(H, precision=prec)
+        assert D.precision == prec and D.stored == M.stored
+        xr = torch.as_tensor(x.real, device="cuda").to(dt).contiguous()
+        xi = torch.as_tensor(x.imag, device="cuda").to(dt).contiguous()
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            capi.check(capi.lib().dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(),
+                                                 s.cuda_stream))
+        s.synchronize()
+        got = (yr.double() + 1j * yi.double()).cpu().numpy()
+        assert rel(got, ref) < tol, prec
+        for bands in (2, 3, 8):  # column-band passes: same values up to summation order
+            capi.check(capi.lib().dq_spmv_device_banded(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(),
+                                                        yi.data_ptr(), bands, 0))
+            torch.cuda.synchronize()
+            assert rel((yr.double() + 1j * yi.double()).cpu().numpy(), ref) < tol, (prec, bands)
+
+
+def test_spmv_full_size_property_2p24(d):
+    """N = 2^24 with 64 random offsets (the SpMV benchmark shape) -- values from closed-form functions evaluated on
+    both sides; rows are checked on a sample against the oracle rule, and linearity on the whole vector."""
+    import torch
+    from diaq_amd import capi
+    N = 1 << 24
+    offs = Hm.random_offsets(N, 64, seed=0)
+    lens = [O.diagonal_length(o, N, N) for o in offs]
+    assert sum(lens) == 798_254_616
+    out = ctypes.c_void_p()
+    o32 = np.asarray(offs, np.int32)
+    capi.check(capi.lib().dq_devmat_zeros(N, N, len(offs), capi.ptr(o32, ctypes.c_int32), capi.DQ_F64,
+                                          ctypes.byref(out)))
+    try:
+        total = sum(lens)
+        pos = torch.cat([torch.arange(L, device="cuda", dtype=torch.float64) for L in lens])
+        kid = torch.cat([torch.full((L,), k, device="cuda", dtype=torch.float64) for k, L in enumerate(lens)])
+        re = torch.cos(1e-3 * pos + 0.7 * kid)
+        im = torch.sin(3e-4 * pos - 0.3 * kid)
+        del pos, kid
+        capi.check(capi.lib().dq_devmat_set_packed_device(out, re.data_ptr(), im.data_ptr(), 0))
+        torch.cuda.synchronize()
+        del re, im
+        idx = torch.arange(N, device="cuda", dtype=torch.float64)
+        x1r, x1i = torch.cos(2e-5 * idx), torch.sin(5e-6 * idx)
+        x2r, x2i = torch.sin(7e-6 * idx + 1.0), torch.cos(1e-6 * idx)
+        ys = []
+        for xr, xi in ((x1r, x1i), (x2r, x2i), (x1r + 2 * x2r, x1i + 2 * x2i)):
+            yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+            capi.check(capi.lib().dq_spmv_device(out, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 0))
+            torch.cuda.synchronize()
+            ys.append((yr, yi))
+        lin_r = ys[2][0] - (ys[0][0] + 2 * ys[1][0])
+        lin_i = ys[2][1] - (ys[0][1] + 2 * ys[1][1])
+        scale = float(torch.max(torch.abs(ys[2][0])))
+        assert float(torch.max(torch.abs(lin_r))) / scale < 1e-12
+        assert float(torch.max(torch.abs(lin_i))) / scale < 1e-12
+        # sampled rows against the row rule with the same closed-form values
+        rows = np.unique(np.concatenate([np.random.default_rng(0).integers(0, N, 4000), [0, N - 1, N // 2]]))
+        x1 = np.cos(2e-5 * np.arange(N)) + 1j * np.sin(5e-6 * np.arange(N))
+        expect = np.zeros(len(rows), complex)
+        for k, o in enumerate(offs):
+            p = rows if o >= 0 else rows + o
+            c = rows + o
+            ok = (p >= 0) & (p < lens[k]) & (c >= 0) & (c < N)
+            a = np.cos(1e-3 * p[ok] + 0.7 * k) + 1j * np.sin(3e-4 * p[ok] - 0.3 * k)
+            expect[ok] += a * x1[c[ok]]
+        got = (ys[0][0] + 1j * ys[0][1]).cpu().numpy()[rows]
+        assert rel(got, expect) < 1e-12
+    finally:
+        capi.lib().dq_devmat_destroy(out)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# SpGEMM
+def test_spgemm_reference_known_answer_pairs(d):
+    g = json.load(open(os.path.join(GOLD, "spgemm_cases.json")))
+    for c in g["cases"]:
+        A = (np.array(c["A"]["re"]) + 1j * np.array(c["A"]["im"])).reshape(c["A"]["shape"])
+        B = (np.array(c["B"]["re"]) + 1j * np.array(c["B"]["im"])).reshape(c["B"]["shape"])
+        P = (np.array(c["product"]["re"]) + 1j * np.array(c["product"]["im"])).reshape(c["product"]["shape"])
+        C = d.product(d.from_numpy(A), d.from_numpy(B))
+        assert list(C.dIndices) == c["product_offsets_pruned"], c["case"]
+        np.testing.assert_allclose(d.to_numpy(C), P, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("shapes,oa,ob", [
+    ((1024, 1024, 1024), [-15, -3, 0, 2, 9, 500], [-500, -9, 0, 1, 3]),
+    ((600, 900, 700), [-599, -20, 0, 33, 899], [-899, -1, 0, 40, 699]),
+    ((4096, 4096, 4096), list(range(-4, 5)), list(range(-4, 5))),
+    ((64, 64, 64), [0, 63, -63], [-63, 63, 1]),
+])
+def test_product_device_structure_and_values(d, shapes, oa, ob):
+    rng = np.random.default_rng(4)
+    m, k, n = shapes
+    A = banded(m, oa, rng, cols=k)
+    B = banded(k, ob, rng, cols=n)
+    MA = O.Dia(m, k, oa, [A[O.position_to_rc(o, np.arange(O.diagonal_length(o, m, k)))] for o in oa])
+    MB = O.Dia(k, n, ob, [B[O.position_to_rc(o, np.arange(O.diagonal_length(o, k, n)))] for o in ob])
+    ref = O.product(MA, MB)
+    C = d.product(d.to_device(dia_of(d, m, k, oa, MA.vals)), d.to_device(dia_of(d, k, n, ob, MB.vals)))
+    assert C.diagonals == ref.offsets  # structural result, no pruning
+    H = C.to_host()
+    assert rel(d.to_numpy(H), O.to_dense(ref)) < TOL
+    Pr = d.prune(C)
+    assert Pr.diagonals == O.prune(ref).offsets
+
+
+def test_prune_constructed_cancellation(d):
+    A = d.to_device(d.from_numpy(np.array([[1, 1], [0, 1]], complex)))
+    B = d.to_device(d.from_numpy(np.array([[1, -1], [0, 1]], complex)))
+    C = d.product(A, B)
+    assert C.diagonals == [0, 1]
+    assert d.prune(C).diagonals == [0]
+    assert list(d.product(d.from_numpy(np.array([[1, 1], [0, 1]], complex)),
+                          d.from_numpy(np.array([[1, -1], [0, 1]], complex))).dIndices) == [0]
+
+
+def test_norms_trace(d):
+    rng = np.random.default_rng(5)
+    A = banded(3000, (-1000, -2, 0, 1, 2999), rng)
+    M = O.from_dense(A)
+    D = d.to_device(d.from_numpy(A))
+    assert abs(d.estimate_1norm(D) - O.estimate_1norm(M)) / O.estimate_1norm(M) < TOL
+    assert abs(d.norm(D) - O.frobenius(M)) / O.frobenius(M) < TOL
+    assert abs(d.trace(D) - O.trace(M)) / abs(O.trace(M)) < TOL
+    assert abs(d.trace(d.from_numpy(A)) - np.trace(A)) / abs(np.trace(A)) < TOL
+    rows, cols, offs, vals = Hm.tfim(12, 1.0, 0.7)
+    T = d.to_device(dia_of(d, rows, cols, offs, vals))
+    assert abs(d.estimate_1norm(T) - 19.4) < 1e-12
+
+
+def test_pow_vs_oracle(d):
+    rng = np.random.default_rng(6)
+    small = banded(96, (-2, -1, 0, 1, 3), rng) / 5
+    M = O.from_dense(small)
+    for n in (0, 1, 2, 3, 6, 7, 8, 13):
+        P = d.pow(d.from_numpy(small), n)
+        ref = O.pow_gpu(M, n)
+        assert list(P.dIndices) == ref.offsets, n
+        assert rel(d.to_numpy(P), O.to_dense(ref)) < TOL
+
+
+def test_tfim20_hh_fp64_fp32(d):
+    """TFIM-20 H.H: 727 offsets bit-exact (structural and pruned), values vs the closed form of the diagonal."""
+    rows, cols, offs, vals = Hm.tfim(20, 1.0, 0.7)
+    H = dia_of(d, rows, cols, offs, vals)
+    res_ref, _ = O.product_dryrun(offs, offs, rows, cols)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        D = d.to_device(H, precision=prec)
+        C = d.product(D, D)
+        assert C.diagonals == res_ref and len(res_ref) == 727
+        assert C.stored == 687_341_554
+        out[prec] = C
+    # (H^2)_{rr} = (diag_r)^2 + n h^2 exactly: check the main diagonal of the FP64 result
+    dg = out["fp64"].get_diagonal(0)
+    expect = vals[offs.index(0)] ** 2 + 20 * 0.7 ** 2
+    assert rel(dg, expect) < TOL
+    # FP32 vs FP64: relative Frobenius difference at FP32 rounding level
+    err = d.norm_diff(out["fp32"], out["fp64"]) / d.norm(out["fp64"])
+    assert 0 < err < 1e-6
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# SpME
+@pytest.mark.parametrize("n,t", [(6, 0.1), (8, 0.7), (9, 2.5)])
+def test_expm_negative_vs_oracle(d, n, t):
+    rows, cols, offs, vals = Hm.tfim(n, 1.0, 0.7)
+    M = O.Dia(rows, cols, offs, vals)
+    Ud, roffs, m, s = O.expm_negative_dense(O.to_dense(M), offs, t)
+    U = d.expm_negative(dia_of(d, rows, cols, offs, vals), t)
+    assert list(U.dIndices) == roffs
+    assert rel(d.to_numpy(U), Ud) < TOL
+
+
+def test_expm_negative_tfim12_config(d):
+    """TFIM-12, t = 0.1: s = 1 by the closed form, offsets from the structural chain, values vs dense FP64 expm."""
+    import scipy.linalg
+    rows, cols, offs, vals = Hm.tfim(12, 1.0, 0.7)
+    M = O.Dia(rows, cols, offs, vals)
+    Hd = O.to_dense(M)
+    Ud, roffs, m, s = O.expm_negative_dense(Hd, offs, 0.1)
+    assert s == 1
+    U = d.expm_negative(dia_of(d, rows, cols, offs, vals), 0.1)
+    assert list(U.dIndices) == roffs
+    Ug = d.to_numpy(U)
+    assert rel(Ug, Ud) < TOL
+    assert np.max(np.abs(Ug - scipy.linalg.expm(-1j * 0.1 * Hd))) < 1e-10
+
+
+def test_expm_positive_and_single_diagonal(d):
+    import scipy.linalg
+    rng = np.random.default_rng(7)
+    A = banded(40, (-2, 0, 3), rng) * 0.3
+    M = O.from_dense(A)
+    U = d.expm(d.from_numpy(A))
+    ref = O.expm(M)
+    assert list(U.dIndices) == ref.offsets
+    assert rel(d.to_numpy(U), O.to_dense(ref)) < TOL
+    assert np.max(np.abs(d.to_numpy(U) - scipy.linalg.expm(A))) < 1e-10
+    Dm = banded(50, (0,), rng)
+    np.testing.assert_allclose(d.to_numpy(d.expm_negative(d.from_numpy(Dm), 0.3)),
+                               np.diag(np.exp(-1j * 0.3 * np.diag(Dm))), rtol=1e-13)
+    np.testing.assert_allclose(d.to_numpy(d.expm(d.from_numpy(Dm))), np.diag(np.exp(np.diag(Dm))), rtol=1e-13)
+    S = banded(50, (3,), rng)  # the reference's fast-path quirk for one off-diagonal
+    assert rel(d.to_numpy(d.expm_negative(d.from_numpy(S), 0.5)),
+               O.to_dense(O.expm_negative(O.from_dense(S), 0.5))) < TOL
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# Fused evolution
+def test_hamsim_vs_oracle(d):
+    import scipy.sparse.linalg
+    rows, cols, offs, vals = Hm.xxz(10, seed=1)
+    M = O.Dia(rows, cols, offs, vals)
+    psi = Hm.product_state(10, 2)
+    H = dia_of(d, rows, cols, offs, vals)
+    for t, r, m in ((0.5, 5, 12), (1.0, 3, 30)):
+        got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), t, r, m))
+        ref = O.hamiltonian_simulation(M, psi, t, r, m)
+        assert rel(got, ref) < TOL
+    exact = scipy.sparse.linalg.expm_multiply(-1j * 1.0 * O.to_dense(M), psi)
+    got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 10, 20))
+    assert np.linalg.norm(got - exact) < 1e-10
+    same = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 0))
+    np.testing.assert_array_equal(same, psi)
+
+
+def test_hamsim_device_in_place(d):
+    import torch
+    rows, cols, offs, vals = Hm.xxz(12, seed=3)
+    M = O.Dia(rows, cols, offs, vals)
+    psi = Hm.product_state(12, 4)
+    D = d.to_device(dia_of(d, rows, cols, offs, vals))
+    pr = torch.as_tensor(psi.real, device="cuda").contiguous()
+    pi = torch.as_tensor(psi.imag, device="cuda").contiguous()
+    d.hamsim_device(D, pr.data_ptr(), pi.data_ptr(), 0.6, 3, 15)
+    torch.cuda.synchronize()
+    got = (pr + 1j * pi).cpu().numpy()
+    assert rel(got, O.hamiltonian_simulation(M, psi, 0.6, 3, 15)) < TOL
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# spmm_dia_device: the reference's four shapes x {complex128, float64} (python_tests/test_spmm_dia_device.py:10-39)
+CASES = [(4096, 4096, 64, [-3, -1, 0, 2, 5]), (1000, 1200, 1, [0, 7, -9]), (300, 200, 17, list(range(-40, 41, 3))),
+         (64, 64, 3, list(range(-63, 64)))]
+
+
+@pytest.mark.parametrize("complex_", [True, False])
+@pytest.mark.parametrize("case", CASES)
+def test_spmm_dia_device(d, case, complex_):
+    import torch
+    M, K, R, offs = case
+    rng = np.random.default_rng(8)
+    V = rng.standard_normal((len(offs), M)) + (1j * rng.standard_normal((len(offs), M)) if complex_ else 0)
+    r = np.arange(M)
+    for k, o in enumerate(offs):
+        V[k, ~((r + o >= 0) & (r + o < K))] = 0
+    B = rng.standard_normal((K, R)) + (1j * rng.standard_normal((K, R)) if complex_ else 0)
+    ref = O.spmm_dia(offs, V, M, K, B)
+    dt = torch.complex128 if complex_ else torch.float64
+    tV = torch.as_tensor(V, device="cuda").to(dt).contiguous()
+    tB = torch.as_tensor(B, device="cuda").to(dt).contiguous()
+    tC = torch.empty((M, R), dtype=dt, device="cuda")
+    off = torch.tensor(offs, dtype=torch.int64, device="cuda")
+    d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(), complex_,
+                      torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert rel(tC.cpu().numpy(), ref) < 1e-13

@@ -339,3 +339,13 @@ def test_c_oracle_block_size_rule():
     l1 = os.sysconf("SC_LEVEL1_DCACHE_SIZE") if "SC_LEVEL1_DCACHE_SIZE" in os.sysconf_names else 0
     if l1 == 49152:
         assert b == 968  # floor(floor(49152 * 0.95 / 48) / 8) * 8
+
+
+def test_dense_expm_oracle_matches_dia_oracle():
+    rows, cols, offs, vals = Hm.tfim(5, 1.0, 0.7)
+    M = O.Dia(rows, cols, offs, vals)
+    U = O.expm_negative(M, 0.3)
+    Ud, roffs, m, s = O.expm_negative_dense(O.to_dense(M), offs, 0.3)
+    assert roffs == U.offsets and s == O.scaling_exponent(O.estimate_1norm(M), 0.3, 1.0)
+    np.testing.assert_allclose(Ud, O.to_dense(U), atol=1e-13)
+    np.testing.assert_allclose(Ud, scipy.linalg.expm(-1j * 0.3 * O.to_dense(M)), atol=1e-11)
