@@ -41,7 +41,7 @@ def cpu_threads() -> int:
     return len(os.sched_getaffinity(0))
 
 
-def spmv_section(args, torch, d, capi, Hm):
+def spmv_section(args, torch, d, capi, Hm, fp32=True):
     N = 1 << 24
     offs = Hm.random_offsets(N, 64, seed=0)
     lens = [Hm.stored_elements(N, N, [o]) for o in offs]
@@ -86,6 +86,8 @@ def spmv_section(args, torch, d, capi, Hm):
 
     # FP32 form of the same matrix (precision form): half the bytes
     try:
+        if not fp32:
+            raise RuntimeError("skipped")
         out32 = ctypes.c_void_p()
         capi.check(L.dq_devmat_convert(out, capi.DQ_F32, ctypes.byref(out32)))
         xr32, xi32 = xr.float(), xi.float()
@@ -201,6 +203,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evolution", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling run: the FP64 SpMV steps only, then one plain copy of 2 GiB (a kernel of known "
+                         "size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -213,6 +218,15 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     torch.cuda.set_device(0)
+    if args.profile:
+        args.no_cpu = True
+        sp = spmv_section(args, torch, d, capi, Hm, fp32=False)
+        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")  # 2 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] profile: spmv {sp['ms']:.3f} ms; calibration copy of {src.numel() * 8} bytes done")
+        return
     sp = spmv_section(args, torch, d, capi, Hm)
     evo = None
     if not args.no_evolution:

@@ -10,6 +10,14 @@ import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
+# One HIP runtime per process: torch ships its own libamdhip64 (same soname as /opt/rocm's). Loading torch first makes
+# this library bind to the runtime torch already loaded; loading this library first would pull in the system runtime
+# and leave torch with a second, GPU-less one.
+try:
+    import torch  # noqa: F401
+except ImportError:  # torch is optional: without it the system HIP runtime is used
+    pass
+
 try:
     from . import diaq_gpu  # noqa: F401  (links libdiaq_hip.so next to it)
 except ImportError as e:  # pragma: no cover - depends on the build

@@ -72,6 +72,10 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
+        try:  # share torch's HIP runtime when torch is present (see diaq_amd/__init__.py)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built (python -m diaq_amd.build)")
         L = ctypes.CDLL(LIB_PATH)

@@ -349,3 +349,10 @@ def test_dense_expm_oracle_matches_dia_oracle():
     assert roffs == U.offsets and s == O.scaling_exponent(O.estimate_1norm(M), 0.3, 1.0)
     np.testing.assert_allclose(Ud, O.to_dense(U), atol=1e-13)
     np.testing.assert_allclose(Ud, scipy.linalg.expm(-1j * 0.3 * O.to_dense(M)), atol=1e-11)
+
+
+def test_spmv_benchmark_offsets_reproduce_survey_count():
+    N = 1 << 24
+    offs = Hm.random_offsets(N, 64, seed=0)
+    assert len(set(offs)) == 64 and all(-N // 2 <= o <= N // 2 for o in offs)
+    assert Hm.stored_elements(N, N, offs) == 798_254_616
