@@ -63,6 +63,13 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     sid = stream.cuda_stream
+    # one-time schedule selection for this matrix (outside the timed region, like a library's plan step)
+    best = ctypes.c_float()
+    capi.check(L.dq_spmv_tune(out, 3, sid, ctypes.byref(best)))
+    cfg = [ctypes.c_int() for _ in range(3)]
+    capi.check(L.dq_spmv_tuned_config(out, *[ctypes.byref(c) for c in cfg]))
+    tuned = dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value, tune_ms=best.value)
+    log(f"[bench] spmv tuned schedule: {tuned}")
 
     def step():
         capi.check(L.dq_spmv_device(out, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
@@ -82,7 +89,7 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     bytes_alg = 16 * S + 16 * N + 16 * N
     gbs = bytes_alg / (ms * 1e-3) / 1e9
     log(f"[bench] spmv fp64: {ms:.3f} ms/step  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS:.3f} of peak)  wall {wall:.2f}s")
-    res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg)
+    res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg, tuned=tuned)
 
     # FP32 form of the same matrix (precision form): half the bytes
     try:
@@ -90,6 +97,7 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
             raise RuntimeError("skipped")
         out32 = ctypes.c_void_p()
         capi.check(L.dq_devmat_convert(out, capi.DQ_F32, ctypes.byref(out32)))
+        capi.check(L.dq_spmv_tune(out32, 3, sid, ctypes.byref(best)))
         xr32, xi32 = xr.float(), xi.float()
         yr32, yi32 = torch.empty_like(xr32), torch.empty_like(xi32)
         for _ in range(args.warmup):
@@ -251,7 +259,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "DIA SpMV y=Ax, N=2^24, 64 diagonals at uniform-random offsets in [-N/2, N/2] "
                                "(numpy default_rng seed 0), complex128 split re/im, values and x torch.randn seed 0",
-                   "N": sp["N"], "n_diags": 64, "stored_elements": sp["S"], "compulsory_bytes": sp["bytes"]},
+                   "N": sp["N"], "n_diags": 64, "stored_elements": sp["S"], "compulsory_bytes": sp["bytes"],
+                   "spmv_schedule": sp.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": round(sp["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu

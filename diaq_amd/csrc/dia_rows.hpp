@@ -29,11 +29,24 @@ struct Acc<float> {  // Kahan-compensated FP32 sum
     __device__ inline float get() const { return s; }
 };
 
-template <typename T, int RPT>
+// Loads of the matrix values: plain, or non-temporal (the values are streamed once, so they need not displace x from
+// L2 and the Infinity Cache).
+template <typename T, bool NT>
+__device__ inline T ld_val(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// PIPE = true: each diagonal's 4*RPT loads are issued together (full tiles unconditionally, boundary tiles with
+// clamped addresses), the schedule that keeps the most bytes in flight. PIPE = false: per-row predicated loads, which
+// the compiler issues row after row -- fewer loads in flight per wave but half the registers, so twice the waves;
+// with scattered offsets (no x reuse between diagonals) that variant can fill L2 faster.
+template <typename T, int RPT, bool NT = false, bool PIPE = true>
 __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, const DiagMeta *__restrict__ meta,
                                      const T *__restrict__ are, const T *__restrict__ aim, const T *__restrict__ xr,
                                      const T *__restrict__ xi, int64_t col_lo, int64_t col_hi, Acc<T> (&ar)[RPT],
                                      Acc<T> (&ai)[RPT]) {
+    constexpr int TILE = 256 * RPT;
     for (int k = 0; k < n; ++k) {
         const DiagMeta m = meta[k];
         const int64_t lo = max(m.rlo, col_lo - m.off);
@@ -43,14 +56,49 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
         const T *pi = aim + m.rowbase;
         const T *qr = xr + m.off;
         const T *qi = xi + m.off;
+        if constexpr (!PIPE) {
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int64_t r = tile + threadIdx.x + 256 * j;
+                if (r >= lo && r < hi) {
+                    const T a_r = ld_val<T, NT>(pr + r), a_i = ld_val<T, NT>(pi + r), x_r = qr[r], x_i = qi[r];
+                    ar[j].add(a_r * x_r - a_i * x_i);
+                    ai[j].add(a_r * x_i + a_i * x_r);
+                }
+            }
+            continue;
+        }
+        // All 4*RPT loads of this diagonal are issued before the first use, so a wave keeps them in flight together.
+        T va_r[RPT], va_i[RPT], vx_r[RPT], vx_i[RPT];
+        if (lo <= tile && hi >= tile + TILE) {  // uniform: every row of the tile is on the diagonal
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int64_t r = tile + threadIdx.x + 256 * j;
+                va_r[j] = ld_val<T, NT>(pr + r);
+                va_i[j] = ld_val<T, NT>(pi + r);
+                vx_r[j] = qr[r];
+                vx_i[j] = qi[r];
+            }
+        } else {  // boundary tile: clamp to a valid row, then zero both operands of the rows off the diagonal
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int64_t r = tile + threadIdx.x + 256 * j;
+                const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
+                va_r[j] = ld_val<T, NT>(pr + rc);
+                va_i[j] = ld_val<T, NT>(pi + rc);
+                vx_r[j] = qr[rc];
+                vx_i[j] = qi[rc];
+            }
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int64_t r = tile + threadIdx.x + 256 * j;
+                if (r < lo || r >= hi) { va_r[j] = T(0); va_i[j] = T(0); vx_r[j] = T(0); vx_i[j] = T(0); }
+            }
+        }
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
-            const int64_t r = tile + threadIdx.x + 256 * j;
-            if (r >= lo && r < hi) {
-                const T a_r = pr[r], a_i = pi[r], x_r = qr[r], x_i = qi[r];
-                ar[j].add(a_r * x_r - a_i * x_i);
-                ai[j].add(a_r * x_i + a_i * x_r);
-            }
+            ar[j].add(va_r[j] * vx_r[j] - va_i[j] * vx_i[j]);
+            ai[j].add(va_r[j] * vx_i[j] + va_i[j] * vx_r[j]);
         }
     }
 }

@@ -133,6 +133,9 @@ struct dq_devmat {
     int64_t total = 0;                // allocated elements per value array (padding included)
     dq::DevBuf re, im;                // values
     dq::DevBuf d_meta;                // DiagMeta[n]
+    // SpMV schedule chosen by dq_spmv_tune for this matrix (0 = the library default): rows per lane, non-temporal
+    // matrix loads (-1 = default), load schedule (-1 = default, 0 = per-row predicated, 1 = batched).
+    int tuned_rpt = 0, tuned_nt = -1, tuned_pipe = -1;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {

@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, c
     const int64_t tile = static_cast<int64_t>(blockIdx.x) * TILE;
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
     Acc<T> hr[RPT], hi[RPT];
-    dia_rows_tile<T, RPT>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi);
+    dia_rows_tile<T, RPT, true, true>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi);
     const T s = static_cast<T>(scale);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
@@ -68,7 +68,7 @@ static void taylor_step(const dq_devmat &H, const T *pr, const T *pi, T *outr, T
         DQ_LAUNCH_CHECK();
         return;
     }
-    constexpr int RPT = sizeof(T) == 8 ? 2 : 4;
+    constexpr int RPT = 8;
     const unsigned grid = blocks_for(N, 256 * RPT, 1LL << 31);
     const T *inr = pr, *ini = pi;
     T *bufr[2] = {t0r, t1r}, *bufi[2] = {t0i, t1i};

@@ -85,19 +85,40 @@ __global__ __launch_bounds__(256) void product_kernel(const WorkItem *__restrict
     T acc_r[PROD_RPT], acc_i[PROD_RPT];
 #pragma unroll
     for (int j = 0; j < PROD_RPT; ++j) { acc_r[j] = T(0); acc_i[j] = T(0); }
+    const int64_t r_end = it.r0 + it.nrows;
     for (int64_t p = p0; p < p1; ++p) {
         const DiagMeta ta = ma[pa[p]];
         const DiagMeta tb = mb[pb[p]];
+        // rows r of this item with r on A's diagonal and r + dA on B's: [lo, hi)
+        const int64_t lo = max(max(ta.rlo, tb.rlo - ta.off), it.r0);
+        const int64_t hi = min(min(ta.rhi, tb.rhi - ta.off), r_end);
+        if (hi <= lo) continue;  // uniform
+        const T *apr = are + ta.rowbase, *api = aim + ta.rowbase;
+        const T *bpr = bre + tb.rowbase + ta.off, *bpi = bim + tb.rowbase + ta.off;
+        T va_r[PROD_RPT], va_i[PROD_RPT], vb_r[PROD_RPT], vb_i[PROD_RPT];
+        if (lo == it.r0 && hi == it.r0 + PROD_CHUNK) {  // uniform: the whole item, no per-row checks
+#pragma unroll
+            for (int j = 0; j < PROD_RPT; ++j) {
+                const int64_t r = it.r0 + threadIdx.x + 256 * j;
+                va_r[j] = apr[r]; va_i[j] = api[r]; vb_r[j] = bpr[r]; vb_i[j] = bpi[r];
+            }
+        } else {  // clamp to a valid row, zero both operands of the rows outside [lo, hi)
+#pragma unroll
+            for (int j = 0; j < PROD_RPT; ++j) {
+                const int64_t r = it.r0 + threadIdx.x + 256 * j;
+                const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
+                va_r[j] = apr[rc]; va_i[j] = api[rc]; vb_r[j] = bpr[rc]; vb_i[j] = bpi[rc];
+            }
+#pragma unroll
+            for (int j = 0; j < PROD_RPT; ++j) {
+                const int64_t r = it.r0 + threadIdx.x + 256 * j;
+                if (r < lo || r >= hi) { va_r[j] = T(0); va_i[j] = T(0); vb_r[j] = T(0); vb_i[j] = T(0); }
+            }
+        }
 #pragma unroll
         for (int j = 0; j < PROD_RPT; ++j) {
-            const int64_t r = it.r0 + threadIdx.x + 256 * j;
-            const int64_t k = r + ta.off;  // row of B
-            if (r < it.r0 + it.nrows && r >= ta.rlo && r < ta.rhi && k >= tb.rlo && k < tb.rhi) {
-                const T a_r = are[ta.rowbase + r], a_i = aim[ta.rowbase + r];
-                const T b_r = bre[tb.rowbase + k], b_i = bim[tb.rowbase + k];
-                acc_r[j] += a_r * b_r - a_i * b_i;
-                acc_i[j] += a_r * b_i + a_i * b_r;
-            }
+            acc_r[j] += va_r[j] * vb_r[j] - va_i[j] * vb_i[j];
+            acc_i[j] += va_r[j] * vb_i[j] + va_i[j] * vb_r[j];
         }
     }
 #pragma unroll

@@ -27,8 +27,10 @@ namespace dq {
 // Process-wide tuning knobs (set through dq_set_tuning).
 static int g_spmv_rpt = 0;    // rows per lane; 0 = automatic
 static int g_spmv_bands = 0;  // column bands; 0 = automatic
+static int g_spmv_nt = -1;    // non-temporal matrix loads: -1 = automatic, 0 = off, 1 = on
+static int g_spmv_pipe = -1;  // load schedule: -1 = automatic, 0 = per-row predicated, 1 = batched
 
-template <typename T, int RPT>
+template <typename T, int RPT, bool NT, bool PIPE>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
                                                         const T *__restrict__ are, const T *__restrict__ aim,
                                                         const T *__restrict__ xr, const T *__restrict__ xi,
@@ -45,7 +47,7 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, con
             if (r < rows) { ar[j].add(yr[r]); ai[j].add(yi[r]); }
         }
     }
-    dia_rows_tile<T, RPT>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai);
+    dia_rows_tile<T, RPT, NT, PIPE>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int64_t r = tile + threadIdx.x + 256 * j;
@@ -54,23 +56,30 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, con
 }
 
 template <typename T>
-static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *yi, int rpt, int64_t c0, int64_t c1,
-                        int accumulate, hipStream_t s) {
+static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *yi, int rpt, bool nt, bool pipe,
+                        int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
     const int64_t rows = A.rows;
     const DiagMeta *meta = A.d_meta.as<DiagMeta>();
     const T *are = static_cast<const T *>(A.re.p), *aim = static_cast<const T *>(A.im.p);
-#define DQ_SPMV_CASE(R)                                                                                           \
-    case R:                                                                                                       \
-        hipLaunchKernelGGL((spmv_rows_kernel<T, R>), dim3(blocks_for(rows, 256 * R, 1LL << 31)), dim3(256), 0, s, \
-                           rows, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate);                      \
+#define DQ_SPMV_LAUNCH(R, NT, PIPE)                                                                           \
+    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE>), dim3(blocks_for(rows, 256 * R, 1LL << 31)), dim3(256), \
+                       0, s, rows, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate)
+#define DQ_SPMV_CASE(R)                                         \
+    case R:                                                     \
+        if (nt && pipe) DQ_SPMV_LAUNCH(R, true, true);          \
+        else if (nt) DQ_SPMV_LAUNCH(R, true, false);            \
+        else if (pipe) DQ_SPMV_LAUNCH(R, false, true);          \
+        else DQ_SPMV_LAUNCH(R, false, false);                   \
         break;
     switch (rpt) {
         DQ_SPMV_CASE(1)
         DQ_SPMV_CASE(2)
         DQ_SPMV_CASE(4)
         DQ_SPMV_CASE(8)
-        default: throw ArgError("spmv: rows per lane must be 1, 2, 4 or 8");
+        DQ_SPMV_CASE(16)
+        default: throw ArgError("spmv: rows per lane must be 1, 2, 4, 8 or 16");
     }
+#undef DQ_SPMV_LAUNCH
 #undef DQ_SPMV_CASE
     DQ_LAUNCH_CHECK();
 }
@@ -90,7 +99,20 @@ static int auto_bands(const dq_devmat &A) {
 
 static int auto_rpt(const dq_devmat &A) {
     if (g_spmv_rpt > 0) return g_spmv_rpt;
-    return A.precision == DQ_F64 ? 2 : 4;
+    if (A.tuned_rpt > 0) return A.tuned_rpt;
+    return 8;
+}
+
+static bool auto_nt(const dq_devmat &A) {
+    if (g_spmv_nt >= 0) return g_spmv_nt == 1;
+    if (A.tuned_nt >= 0) return A.tuned_nt == 1;
+    return true;
+}
+
+static bool auto_pipe(const dq_devmat &A) {
+    if (g_spmv_pipe >= 0) return g_spmv_pipe == 1;
+    if (A.tuned_pipe >= 0) return A.tuned_pipe == 1;
+    return true;
 }
 
 void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, int n_bands,
@@ -110,15 +132,16 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     if (n_bands <= 0) n_bands = auto_bands(A);
     if (n_bands > static_cast<int>(A.cols)) n_bands = std::max<int>(1, A.cols);
     const int rpt = auto_rpt(A);
+    const bool nt = auto_nt(A), pipe = auto_pipe(A);
     const int64_t cols = A.cols;
     for (int b = 0; b < n_bands; ++b) {
         const int64_t c0 = cols * b / n_bands, c1 = cols * (b + 1) / n_bands;
         if (A.precision == DQ_F64)
-            launch_rows<double>(A, (const double *)xr, (const double *)xi, (double *)yr, (double *)yi, rpt, c0, c1,
-                                b > 0, s);
+            launch_rows<double>(A, (const double *)xr, (const double *)xi, (double *)yr, (double *)yi, rpt, nt, pipe,
+                                c0, c1, b > 0, s);
         else
-            launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, c0, c1, b > 0,
-                               s);
+            launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, nt, pipe, c0,
+                               c1, b > 0, s);
     }
 }
 
@@ -135,15 +158,77 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         if (!key) throw ArgError("key is NULL");
         const std::string k(key);
         if (k == "spmv_rows_per_lane") {
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
-                throw ArgError("spmv_rows_per_lane must be 0, 1, 2, 4 or 8");
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                throw ArgError("spmv_rows_per_lane must be 0, 1, 2, 4, 8 or 16");
             g_spmv_rpt = value;
+        } else if (k == "spmv_nontemporal") {
+            if (value < -1 || value > 1) throw ArgError("spmv_nontemporal must be -1, 0 or 1");
+            g_spmv_nt = value;
+        } else if (k == "spmv_pipelined") {
+            if (value < -1 || value > 1) throw ArgError("spmv_pipelined must be -1, 0 or 1");
+            g_spmv_pipe = value;
         } else if (k == "spmv_bands") {
             if (value < 0 || value > 4096) throw ArgError("spmv_bands must be in [0, 4096]");
             g_spmv_bands = value;
         } else {
             throw ArgError("unknown tuning key: " + k);
         }
+    });
+}
+
+// Time every (load schedule, rows per lane, non-temporal) candidate on scratch vectors and keep the fastest in the
+// matrix handle; later SpMVs with this matrix use it (unless a process-wide knob overrides it).
+extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms) {
+    return guard([&] {
+        if (!A) throw ArgError("matrix is NULL");
+        if (repeats <= 0) repeats = 3;
+        hipStream_t s = as_stream(stream);
+        if (A->rows == 0 || A->n() == 0) return;
+        const size_t eb = elem_bytes(A->precision);
+        DevBuf xr(A->cols * eb), xi(A->cols * eb), yr(A->rows * eb), yi(A->rows * eb);
+        DQ_HIP(hipMemsetAsync(xr.p, 0, xr.bytes, s));
+        DQ_HIP(hipMemsetAsync(xi.p, 0, xi.bytes, s));
+        hipEvent_t e0, e1;
+        DQ_HIP(hipEventCreate(&e0));
+        DQ_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        int brpt = 8, bnt = 1, bpipe = 1;
+        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe;
+        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1;
+        try {
+            for (int pipe = 0; pipe <= 1; ++pipe)
+                for (int rpt : {2, 4, 8, 16})
+                    for (int nt = 0; nt <= 1; ++nt) {
+                        A->tuned_rpt = rpt; A->tuned_nt = nt; A->tuned_pipe = pipe;
+                        spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);  // warm
+                        DQ_HIP(hipEventRecord(e0, s));
+                        for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);
+                        DQ_HIP(hipEventRecord(e1, s));
+                        DQ_HIP(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        ms /= repeats;
+                        if (ms < best) { best = ms; brpt = rpt; bnt = nt; bpipe = pipe; }
+                    }
+        } catch (...) {
+            g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe;
+            (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+            throw;
+        }
+        g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+extern "C" int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined) {
+    return guard([&] {
+        if (!A) throw ArgError("matrix is NULL");
+        if (rows_per_lane) *rows_per_lane = auto_rpt(*A);
+        if (nontemporal) *nontemporal = auto_nt(*A) ? 1 : 0;
+        if (pipelined) *pipelined = auto_pipe(*A) ? 1 : 0;
     });
 }
 
