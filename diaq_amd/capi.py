@@ -66,6 +66,15 @@ SIGNATURES = {
     "dq_spmm_dia_device": (c_int, [c_uintptr, c_uintptr, c_int, c_int64, c_int64, c_uintptr, c_int64, c_uintptr, c_int,
                                    c_uintptr]),
     "dq_dia_viability": (c_int, [c_int64, c_int64, c_int64, c_int64, c_int64, c_int, c_int, P(DiaViability)]),
+    "dq_evo_create": (c_int, [c_int, P(c_void_p), c_int, c_int, P(c_void_p)]),
+    "dq_evo_destroy": (None, [c_void_p]),
+    "dq_evo_info": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+    "dq_evo_set_states": (c_int, [c_void_p, c_int, P(c_double), P(c_double)]),
+    "dq_evo_get_states": (c_int, [c_void_p, c_int, P(c_double), P(c_double)]),
+    "dq_evo_set_schedule": (c_int, [c_void_p, P(c_double), P(c_int32), P(c_int32)]),
+    "dq_evo_tune": (c_int, [c_void_p, P(c_int)]),
+    "dq_evo_run": (c_int, [c_void_p, c_int, c_int, c_uintptr, P(ctypes.c_float)]),
+    "dq_evo_probe": (c_int, [c_void_p, P(c_double)]),
 }
 
 _LIB = None

@@ -105,3 +105,22 @@ def stored_elements(rows: int, cols: int, offsets) -> int:
         n = rows + d if d < 0 else (cols - d if d > 0 else min(rows, cols))
         s += max(0, min(n, rows, cols))
     return s
+
+
+def batched_tfim_workload(n_ham: int = 256, n_states: int = 8, seed: int = 0, n_min: int = 10, n_max: int = 18):
+    """The batched-evolution workload: n_ham TFIM Hamiltonians with n ~ U{n_min..n_max} qubits, J ~ U[0.5, 1.5],
+    h ~ U[0.2, 2.0], each with n_states random complex64 unit states, evolved to t ~ log-uniform[0.01, 5] (all
+    seeded). Returns a list of dicts (n, J, h, t, states as complex128 holding complex64-rounded values)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_ham):
+        n = int(rng.integers(n_min, n_max, endpoint=True))
+        J = float(rng.uniform(0.5, 1.5))
+        h = float(rng.uniform(0.2, 2.0))
+        t = float(np.exp(rng.uniform(np.log(0.01), np.log(5.0))))
+        N = 1 << n
+        st = rng.standard_normal((n_states, N)) + 1j * rng.standard_normal((n_states, N))
+        st /= np.linalg.norm(st, axis=1, keepdims=True)
+        st = st.astype(np.complex64).astype(np.complex128)
+        jobs.append(dict(n=n, J=J, h=h, t=t, states=st))
+    return jobs

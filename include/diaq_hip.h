@@ -203,6 +203,39 @@ typedef struct {
 int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right, int itemsize,
                      int index_bytes, dq_dia_viability_t *out);
 
+/* ------------------------------------------------------------------------------------------------------------------
+ * Batched time evolution (no reference counterpart: the reference evolves one state of one Hamiltonian per call,
+ * hamiltonian_simulation_cuda, src/spHamSim_gpu_all_fused.cu:287-376)
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* Opaque plan: n_jobs square Hamiltonians (jobs of the same size and diagonal set share one class), n_states states
+ * per job (1, 2, 4, 8 or 16), Hamiltonian values stored in value_precision (DQ_F32 or DQ_F64). States are FP64; the
+ * Taylor terms run in FP32 with compensated row sums. The plan copies the matrices: they may be freed afterwards. */
+typedef struct dq_evo dq_evo;
+int dq_evo_create(int n_jobs, dq_devmat *const *H, int n_states, int value_precision, dq_evo **out);
+void dq_evo_destroy(dq_evo *E);
+int dq_evo_info(const dq_evo *E, int *n_classes, int *n_jobs, int *n_states);
+
+/* States of one job, state-major host arrays: state s, row r at [s*N + r]. */
+int dq_evo_set_states(dq_evo *E, int job, const double *re, const double *im);
+int dq_evo_get_states(const dq_evo *E, int job, double *re, double *im);
+
+/* Per job: evolve to t[j] by 2^s[j] sub-steps of a degree-m[j] Taylor polynomial (1 <= m <= 64, 0 <= s <= 24). */
+int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int32_t *s);
+
+/* Rows per lane of every class by timing the candidates (after the schedule is set); chosen (may be NULL) receives
+ * one value per class. */
+int dq_evo_tune(dq_evo *E, int *chosen);
+
+/* A-posteriori truncation estimate per job from its current (initial) states: 2^s ||t_{m+1}|| / (1 - ratio) of the
+ * first sub-step's Taylor terms, the largest over the job's states. Leaves the states unchanged. */
+int dq_evo_probe(dq_evo *E, double *est);
+
+/* Evolve every job by its schedule; use_graph != 0 captures the whole batch (one stream per class) into a HIP graph
+ * on the first call and replays it; renormalize != 0 ends with an FP64 renormalisation of every state. ms_out (may
+ * be NULL) receives the GPU time. */
+int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t stream, float *ms_out);
+
 #ifdef __cplusplus
 }
 #endif

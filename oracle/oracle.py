@@ -442,6 +442,22 @@ def hamiltonian_simulation(H: Dia, psi0: np.ndarray, t: float, r: int, max_itera
     return psi
 
 
+def taylor_action_ms(H: np.ndarray, psi: np.ndarray, t: float, m: int, s: int) -> np.ndarray:
+    """[T_m(Y)]^(2^s) psi with Y = -i (t / 2^s) H and T_m the degree-m Taylor polynomial of exp: the approximation
+    a scheduled (m, s) evolution computes (the per-step action of src/spHamSim_gpu_all_fused.cu:235-277 with
+    2^s steps of t / 2^s and degree m). Dense FP64 (H may be a dense or scipy sparse matrix)."""
+    tau = math.ldexp(t, -s)
+    out = np.array(psi, dtype=np.complex128, copy=True)
+    for _ in range(1 << s):
+        term = out.copy()
+        acc = out.copy()
+        for k in range(1, m + 1):
+            term = (-1j * tau / k) * (H @ term)
+            acc = acc + term
+        out = acc
+    return out
+
+
 # ----------------------------------------------------------------------------------------------------------------------
 # DIA x dense and the viability rule
 # ----------------------------------------------------------------------------------------------------------------------

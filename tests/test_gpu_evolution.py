@@ -1,0 +1,92 @@
+"""GPU tests of the batched evolution plan (FP32 Taylor terms, FP64 states, HIP graph) against the FP64 oracle."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as O  # noqa: E402
+from diaq_amd import hamiltonians as Hm  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import diaq_amd  # noqa: F401
+    import diaq_gpu
+    from diaq_amd import evolution
+    return diaq_gpu, evolution
+
+
+def small_batch(d, seed=0, n_states=4):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for n in (6, 7, 8, 6, 8, 9):
+        J, h = rng.uniform(0.5, 1.5), rng.uniform(0.2, 2.0)
+        t = float(np.exp(rng.uniform(np.log(0.05), np.log(2.0))))
+        r, c, offs, vals = Hm.tfim(n, J, h)
+        D = d.to_device(d.diaQMatrix.from_diagonals([r, c], offs, vals))
+        st = rng.standard_normal((n_states, r)) + 1j * rng.standard_normal((n_states, r))
+        st /= np.linalg.norm(st, axis=1, keepdims=True)
+        jobs.append(dict(n=n, t=t, D=D, dense=O.to_dense(O.Dia(r, c, offs, vals)), states=st))
+    return jobs
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batched_vs_dense_expm_and_graph_replays(mods, prec):
+    d, ev = mods
+    jobs = small_batch(d)
+    tol = 1e-8
+    sched = [ev.plain_rule(d.estimate_1norm(j["D"]), j["t"], tol) for j in jobs]
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=4, value_precision=prec)
+    assert B.n_classes == 4  # sizes 6, 7, 8, 9
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in sched], [s for m, s in sched])
+
+    def load():
+        for q, j in enumerate(jobs):
+            B.set_states(q, j["states"])
+
+    load()
+    B.run(use_graph=False, renormalize=True)
+    eager = [B.get_states(q, 1 << j["n"]) for q, j in enumerate(jobs)]
+    worst = 0.0
+    for j, got in zip(jobs, eager):
+        U = scipy.linalg.expm(-1j * j["t"] * j["dense"])
+        for st, g in zip(j["states"], got):
+            worst = max(worst, np.linalg.norm(g - U @ st))
+    assert worst < 1e-6, worst
+    for _ in range(3):  # graph replays, each from the initial states, each equal to the eager run
+        load()
+        B.run(use_graph=True, renormalize=True)
+        for q, j in enumerate(jobs):
+            np.testing.assert_array_equal(B.get_states(q, 1 << j["n"]), eager[q])
+
+
+def test_batched_matches_taylor_action_oracle_for_any_schedule(mods):
+    d, ev = mods
+    jobs = small_batch(d, seed=1, n_states=2)
+    ms = [(3, 1), (5, 0), (2, 3), (7, 2), (1, 4), (4, 2)]
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=2, value_precision="fp64")
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in ms], [s for m, s in ms])
+    for q, j in enumerate(jobs):
+        B.set_states(q, j["states"])
+    B.run(use_graph=True, renormalize=False)
+    for q, (j, (m, s)) in enumerate(zip(jobs, ms)):
+        got = B.get_states(q, 1 << j["n"])
+        for st, g in zip(j["states"], got):
+            ref = O.taylor_action_ms(j["dense"], st, j["t"], m, s)
+            assert np.linalg.norm(g - ref) / np.linalg.norm(ref) < 2e-6
+
+
+def test_probe_flags_too_small_degree(mods):
+    d, ev = mods
+    jobs = small_batch(d, seed=2)
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=4)
+    good = [ev.plain_rule(d.estimate_1norm(j["D"]), j["t"], 1e-8) for j in jobs]
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in good], [s for m, s in good])
+    for q, j in enumerate(jobs):
+        B.set_states(q, j["states"])
+    est_good = B.probe()
+    assert np.all(est_good < 1e-6), est_good
+    B.set_schedule([j["t"] for j in jobs], [2] * len(jobs), [0] * len(jobs))
+    est_bad = B.probe()
+    assert np.all(est_bad > 1e-6), est_bad

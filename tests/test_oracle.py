@@ -356,3 +356,19 @@ def test_spmv_benchmark_offsets_reproduce_survey_count():
     offs = Hm.random_offsets(N, 64, seed=0)
     assert len(set(offs)) == 64 and all(-N // 2 <= o <= N // 2 for o in offs)
     assert Hm.stored_elements(N, N, offs) == 798_254_616
+
+
+def test_taylor_action_ms_oracle():
+    rows, cols, offs, vals = Hm.tfim(5, 1.1, 0.9)
+    H = O.to_dense(O.Dia(rows, cols, offs, vals))
+    psi = Hm.random_state(rows, 1)
+    exact = scipy.linalg.expm(-1j * 1.3 * H) @ psi
+    np.testing.assert_allclose(O.taylor_action_ms(H, psi, 1.3, 30, 3), exact, atol=1e-13)
+    # one step, degree m: the truncated series on each eigencomponent (closed form)
+    w, V = np.linalg.eigh(H)
+    c = V.conj().T @ psi
+    from math import factorial
+    poly = sum((-1j * 0.4 * w) ** k / factorial(k) for k in range(0, 4))
+    np.testing.assert_allclose(O.taylor_action_ms(H, psi, 0.4, 3, 0), V @ (poly * c), atol=1e-14)
+    poly2 = sum((-1j * 0.2 * w) ** k / factorial(k) for k in range(0, 3)) ** 2
+    np.testing.assert_allclose(O.taylor_action_ms(H, psi, 0.4, 2, 1), V @ (poly2 * c), atol=1e-14)
