@@ -7,10 +7,11 @@ everything resident in HBM. A step is one SpMV; value = compulsory bytes (16*S +
 per second of the K timed steps, in GB/s; roofline.frac = that over the MI355X HBM peak (8 TB/s).
 
 Also reported: the reference-style CPU SpMV (the C oracle's restatement of the reference's OpenMP/SIMD path) timed on
-this host on the same matrix, and the time evolution of the 20-qubit XXZ chain (100 steps, dt = 0.05) by the plain
-FP64 fused path.
+this host on the same matrix; the time evolution of the 20-qubit XXZ chain (100 steps, dt = 0.05) by the plain FP64
+fused path; and the batched evolution of 256 TFIM Hamiltonians x 8 states -- evolutions/s of the batched FP32 plan
+with learned (m, s) against the plain FP64 path, with state errors against an FP64 reference.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-evolution]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-evolution] [--no-batched]
 """
 from __future__ import annotations
 
@@ -204,6 +205,105 @@ def evolution_section(args, torch, d, capi, Hm):
     return out
 
 
+def batched_section(args, torch, d, capi, Hm):
+    """256 TFIM Hamiltonians (n ~ U{10..18}, J ~ U[0.5, 1.5], h ~ U[0.2, 2]) x 8 complex64 unit states, each job
+    evolved to t ~ log-uniform[0.01, 5]: the plain FP64 path (one fused evolution per state, (m, s) by the plain norm
+    rule) against the batched FP32 plan with learned (m, s) (probe, fallback to the plain rule), both against an FP64
+    reference (the plain FP64 path at tol 1e-12)."""
+    from diaq_amd import evolution as ev
+    from diaq_amd import selector as S
+    jobs = Hm.batched_tfim_workload(256, 8, seed=0)
+    Ds = []
+    for j in jobs:
+        r, c, offs, vals = Hm.tfim(j["n"], j["J"], j["h"])
+        Ds.append(d.to_device(d.diaQMatrix.from_diagonals([r, c], offs, vals)))
+    n_ev = 8 * len(jobs)
+    tol = 1e-6
+    norms = [d.estimate_1norm(D) for D in Ds]
+    plain = [ev.plain_rule(nm, j["t"], tol) for nm, j in zip(norms, jobs)]
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def fused(sched, tag):
+        out = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for q, (j, D, (m, s)) in enumerate(zip(jobs, Ds, sched)):
+            res = []
+            for st in j["states"]:
+                pr = torch.as_tensor(st.real, device="cuda").contiguous()
+                pi = torch.as_tensor(st.imag, device="cuda").contiguous()
+                d.hamsim_device(D, pr.data_ptr(), pi.data_ptr(), j["t"], 1 << s, m, sid)
+                res.append(torch.complex(pr, pi))
+            out.append(torch.stack(res))
+            if q % 64 == 0:
+                log(f"[bench] batched workload, {tag}: job {q}/{len(jobs)} {time.perf_counter() - t0:.1f}s")
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    ref, t_ref = fused([ev.plain_rule(nm, j["t"], 1e-12) for nm, j in zip(norms, jobs)], "FP64 reference")
+    plain_out, t_plain = fused(plain, "plain FP64")
+    err_plain = max(float(torch.max(torch.linalg.vector_norm(a - b, dim=1))) for a, b in zip(plain_out, ref))
+    log(f"[bench] plain FP64 path: {t_plain:.2f}s for {n_ev} evolutions, max state error {err_plain:.2e}")
+
+    sel = S.Selector.load()
+    B = ev.BatchedEvolution(Ds, 8, "fp32")
+
+    def load():
+        for q, j in enumerate(jobs):
+            B.set_states(q, j["states"])
+
+    def learned_run():
+        feats = np.array([S.features(nm, j["t"], len(D.diagonals), max(abs(o) for o in D.diagonals), D.shape[0],
+                                     tol / 2) for nm, j, D in zip(norms, jobs, Ds)])
+        prop = sel.predict(feats)
+        B.set_schedule([j["t"] for j in jobs], [m for m, s in prop], [s for m, s in prop])
+        est = B.probe()
+        final = [p if e <= tol / 2 else q for p, q, e in zip(prop, plain, est)]
+        B.set_schedule([j["t"] for j in jobs], [m for m, s in final], [s for m, s in final])
+        ms = B.run(use_graph=True, renormalize=True)
+        return prop, final, est, ms
+
+    load()
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in plain], [s for m, s in plain])
+    tuned = B.tune()
+    load()
+    learned_run()  # warm-up: builds the graph once
+    load()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prop, final, est, gpu_ms = learned_run()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    out_b = [torch.as_tensor(B.get_states(q, 1 << j["n"]), device="cuda") for q, j in enumerate(jobs)]
+    err_b = max(float(torch.max(torch.linalg.vector_norm(a - b, dim=1))) for a, b in zip(out_b, ref))
+    fallbacks = int(sum(1 for p, f in zip(prop, final) if p != f))
+    spmv_plain = int(sum(m * 2 ** s for m, s in plain)) * 8
+    spmv_learned = int(sum(m * 2 ** s for m, s in final)) * 8
+    # the same plan with the plain schedule, to separate the batching/precision gain from the selector's
+    load()
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in plain], [s for m, s in plain])
+    B.run(use_graph=True)
+    load()
+    t1 = time.perf_counter()
+    gpu_ms_plain_sched = B.run(use_graph=True)
+    t_b_plain = time.perf_counter() - t1
+    log(f"[bench] batched FP32 + learned (m,s): {t_batched:.3f}s ({gpu_ms:.1f} ms graph), max state error "
+        f"{err_b:.2e}, fallbacks {fallbacks}; same plan with the plain schedule {t_b_plain:.3f}s")
+    return dict(
+        workload="256 TFIM (n~U{10..18}, J~U[0.5,1.5], h~U[0.2,2.0]) x 8 complex64 unit states, t~logU[0.01,5], "
+                 "seed 0",
+        evolutions=n_ev, tol=tol,
+        plain_fp64=dict(seconds=t_plain, evolutions_per_s=n_ev / t_plain, max_state_error=err_plain,
+                        spmv_per_state_total=spmv_plain),
+        batched_fp32_learned=dict(seconds=t_batched, graph_ms=gpu_ms, evolutions_per_s=n_ev / t_batched,
+                                  max_state_error=err_b, fallbacks=fallbacks, spmv_total=spmv_learned,
+                                  tuned_rows_per_lane=tuned),
+        batched_fp32_plain_schedule=dict(seconds=t_b_plain, graph_ms=gpu_ms_plain_sched,
+                                         evolutions_per_s=n_ev / t_b_plain),
+        speedup_vs_plain_fp64=t_plain / t_batched,
+        reference="plain FP64 fused path at tol 1e-12", reference_seconds=t_ref)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +311,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evolution", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: the FP64 SpMV steps only, then one plain copy of 2 GiB (a kernel of known "
                          "size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
@@ -243,6 +344,13 @@ def main():
         except Exception as e:
             evo = dict(error=str(e))
             log(f"[bench] evolution failed: {e}")
+    bat = None
+    if not args.no_batched:
+        try:
+            bat = batched_section(args, torch, d, capi, Hm)
+        except Exception as e:
+            bat = dict(error=str(e))
+            log(f"[bench] batched section failed: {e}")
     cpu = sp.get("cpu", {})
     line = {
         "metric": METRIC,
@@ -268,6 +376,7 @@ def main():
         "spmv_fp32": sp.get("fp32"),
         "cpu_detail": cpu or None,
         "evolution": evo,
+        "batched_evolution": bat,
     }
     print(json.dumps(line), flush=True)
 

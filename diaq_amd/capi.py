@@ -75,6 +75,15 @@ SIGNATURES = {
     "dq_evo_tune": (c_int, [c_void_p, P(c_int)]),
     "dq_evo_run": (c_int, [c_void_p, c_int, c_int, c_uintptr, P(ctypes.c_float)]),
     "dq_evo_probe": (c_int, [c_void_p, P(c_double)]),
+    "dq_mlp_create": (c_int, [c_int, c_int, c_int, c_int, c_uint64, P(c_void_p)]),
+    "dq_mlp_destroy": (None, [c_void_p]),
+    "dq_mlp_num_params": (c_int64, [c_void_p]),
+    "dq_mlp_get_params": (c_int, [c_void_p, P(ctypes.c_float)]),
+    "dq_mlp_set_params": (c_int, [c_void_p, P(ctypes.c_float)]),
+    "dq_mlp_forward": (c_int, [c_void_p, c_int, c_void_p, P(ctypes.c_float)]),
+    "dq_mlp_loss_grad": (c_int, [c_void_p, c_int, c_void_p, c_void_p, P(ctypes.c_float), P(ctypes.c_float)]),
+    "dq_mlp_train_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, ctypes.c_float, P(ctypes.c_float)]),
+    "dq_mlp_predict": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_uintptr]),
 }
 
 _LIB = None

@@ -124,3 +124,41 @@ def batched_tfim_workload(n_ham: int = 256, n_states: int = 8, seed: int = 0, n_
         st = st.astype(np.complex64).astype(np.complex128)
         jobs.append(dict(n=n, J=J, h=h, t=t, states=st))
     return jobs
+
+
+def pauli_sum(n_qubits: int, n_strings: int, rng, min_weight: int = 2, max_weight: int = 4):
+    """Random Hermitian Pauli sum H = sum_p c_p P_p with c_p ~ N(0, 1) and each P_p acting as X, Y or Z on
+    min_weight..max_weight distinct qubits (identity elsewhere), built directly in DIA.
+
+    P|r> = phase(r) |r xor f> with f the mask of X/Y qubits and phase(r) = i^{#Y} (-1)^{popcount(r & (Z|Y mask))}
+    (Y|0> = i|1>, Y|1> = -i|0>), so H[r xor f, r] += c phase(r): the entry sits on diagonal d = r - (r xor f),
+    one of up to 2^{|f|} offsets. Returns (N, N, offsets ascending, vals, strings) with strings as (coef, ops dict).
+    """
+    N = 1 << n_qubits
+    r = np.arange(N, dtype=np.int64)
+    acc = {}
+    strings = []
+    for _ in range(n_strings):
+        w = int(rng.integers(min_weight, max_weight, endpoint=True))
+        qs = rng.choice(n_qubits, size=w, replace=False)
+        ops = {int(q): "XYZ"[int(rng.integers(0, 3))] for q in qs}
+        c = float(rng.standard_normal())
+        strings.append((c, ops))
+        f = sum(1 << q for q, o in ops.items() if o in "XY")
+        zy = sum(1 << q for q, o in ops.items() if o in "YZ")
+        n_y = sum(1 for o in ops.values() if o == "Y")
+        par = np.bitwise_count(r & zy) & 1
+        phase = (1j ** n_y) * (1.0 - 2.0 * par)
+        rows = r ^ f
+        dvec = r - rows
+        order = np.argsort(dvec, kind="stable")
+        ds = dvec[order]
+        cuts = np.flatnonzero(np.diff(ds)) + 1
+        for grp in np.split(order, cuts):
+            dv = int(dvec[grp[0]])
+            pos = rows[grp] if dv >= 0 else r[grp]
+            if dv not in acc:
+                acc[dv] = np.zeros(N - abs(dv), np.complex128)
+            acc[dv][pos] += c * phase[grp]  # r -> r xor f is one-to-one: no repeated positions
+    offs = sorted(d for d, a in acc.items() if np.any(a != 0))
+    return N, N, offs, [acc[d] for d in offs], strings

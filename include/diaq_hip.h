@@ -236,6 +236,26 @@ int dq_evo_probe(dq_evo *E, double *est);
  * be NULL) receives the GPU time. */
 int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t stream, float *ms_out);
 
+/* ------------------------------------------------------------------------------------------------------------------
+ * The learned (m, s) selector's network: an FP32 MLP n_in -> h1 -> h2 -> n_out (tanh hidden layers, softmax
+ * output) trained and run by hand-written HIP kernels. No reference counterpart.
+ * Parameter layout (flat, FP32): W1 [h1 x n_in], b1 [h1], W2 [h2 x h1], b2 [h2], W3 [n_out x h2], b3 [n_out].
+ * ---------------------------------------------------------------------------------------------------------------- */
+typedef struct dq_mlp dq_mlp;
+int dq_mlp_create(int n_in, int h1, int h2, int n_out, uint64_t seed, dq_mlp **out); /* Glorot-uniform init */
+void dq_mlp_destroy(dq_mlp *M);
+int64_t dq_mlp_num_params(const dq_mlp *M);
+int dq_mlp_get_params(const dq_mlp *M, float *host);
+int dq_mlp_set_params(dq_mlp *M, const float *host); /* also resets the optimiser state */
+/* x_dev: B x n_in features on the device; logits_host (may be NULL) receives B x n_out logits. */
+int dq_mlp_forward(dq_mlp *M, int B, const float *x_dev, float *logits_host);
+/* Mean softmax cross-entropy of labels y_dev (B int32 classes) and its gradient (grad_host: n_params, may be NULL). */
+int dq_mlp_loss_grad(dq_mlp *M, int B, const float *x_dev, const int32_t *y_dev, float *loss, float *grad_host);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr; loss (may be NULL): the batch loss. */
+int dq_mlp_train_step(dq_mlp *M, int B, const float *x_dev, const int32_t *y_dev, float lr, float *loss);
+/* Argmax class per sample over the classes with mask_dev[o] != 0 (mask_dev may be NULL), into cls_dev (device). */
+int dq_mlp_predict(dq_mlp *M, int B, const float *x_dev, const int32_t *mask_dev, int32_t *cls_dev, uintptr_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -497,6 +497,57 @@ def dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize=16, index_byte
                 csr_bytes=csr_bytes, density=density, diagonal_budget=budget, reason=reason)
 
 
+# ----------------------------------------------------------------------------------------------------------------------
+# The (m, s) selector's network (no reference counterpart): numpy twin of the HIP kernels, for gradient checks
+# ----------------------------------------------------------------------------------------------------------------------
+def mlp_split(params, dims):
+    """Views W1, b1, W2, b2, W3, b3 of a flat parameter vector laid out as the C ABI states."""
+    n_in, h1, h2, n_out = dims
+    sizes = [h1 * n_in, h1, h2 * h1, h2, n_out * h2, n_out]
+    out, o = [], 0
+    for sz in sizes:
+        out.append(params[o:o + sz])
+        o += sz
+    W1, b1, W2, b2, W3, b3 = out
+    return W1.reshape(h1, n_in), b1, W2.reshape(h2, h1), b2, W3.reshape(n_out, h2), b3
+
+
+def mlp_forward(params, dims, x):
+    """logits and hidden activations: a1 = tanh(W1 x + b1), a2 = tanh(W2 a1 + b2), z = W3 a2 + b3 (float64)."""
+    W1, b1, W2, b2, W3, b3 = mlp_split(np.asarray(params, np.float64), dims)
+    x = np.asarray(x, np.float64)
+    a1 = np.tanh(x @ W1.T + b1)
+    a2 = np.tanh(a1 @ W2.T + b2)
+    return a2 @ W3.T + b3, a1, a2
+
+
+def mlp_loss_grad(params, dims, x, y):
+    """Mean softmax cross-entropy over the batch and its gradient, flat in the parameter layout."""
+    x = np.asarray(x, np.float64)
+    z, a1, a2 = mlp_forward(params, dims, x)
+    B = x.shape[0]
+    zm = z - z.max(axis=1, keepdims=True)
+    p = np.exp(zm) / np.exp(zm).sum(axis=1, keepdims=True)
+    loss = float(np.mean(-np.log(p[np.arange(B), y])))
+    d3 = p.copy()
+    d3[np.arange(B), y] -= 1.0
+    d3 /= B
+    W1, b1, W2, b2, W3, b3 = mlp_split(np.asarray(params, np.float64), dims)
+    gW3, gb3 = d3.T @ a2, d3.sum(0)
+    d2 = (d3 @ W3) * (1 - a2 ** 2)
+    gW2, gb2 = d2.T @ a1, d2.sum(0)
+    d1 = (d2 @ W2) * (1 - a1 ** 2)
+    gW1, gb1 = d1.T @ x, d1.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+
+
+def adam_step(params, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """Adam with bias correction (Kingma & Ba 2015), in place on numpy arrays."""
+    m[:] = b1 * m + (1 - b1) * grad
+    v[:] = b2 * v + (1 - b2) * grad * grad
+    params -= lr * (m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps)
+
+
 PINS = {
     "diagonal_length": "brute force over dense masks (tests/test_oracle.py::test_diagonal_length_brute)",
     "padded_offsets": "closed form: starts are multiples of 8 and cover every diagonal",
@@ -512,4 +563,5 @@ PINS = {
     "taylor_action/hamsim": "scipy.linalg.expm_multiply / dense expm; 2-norm preservation",
     "spmm_dia": "dense matmul",
     "dia_viability": "hand-computed budget cases",
+    "mlp_loss_grad": "central finite differences of mlp_forward's loss",
 }

@@ -372,3 +372,23 @@ def test_taylor_action_ms_oracle():
     np.testing.assert_allclose(O.taylor_action_ms(H, psi, 0.4, 3, 0), V @ (poly * c), atol=1e-14)
     poly2 = sum((-1j * 0.2 * w) ** k / factorial(k) for k in range(0, 3)) ** 2
     np.testing.assert_allclose(O.taylor_action_ms(H, psi, 0.4, 2, 1), V @ (poly2 * c), atol=1e-14)
+
+
+def test_mlp_gradient_vs_finite_differences():
+    rng = np.random.default_rng(12)
+    dims = (5, 7, 6, 9)
+    n = 7 * 5 + 7 + 6 * 7 + 6 + 9 * 6 + 9
+    p = rng.standard_normal(n) * 0.5
+    x = rng.standard_normal((11, 5))
+    y = rng.integers(0, 9, 11)
+    loss, g = O.mlp_loss_grad(p, dims, x, y)
+
+    def L(q):
+        z, _, _ = O.mlp_forward(q, dims, x)
+        zm = z - z.max(1, keepdims=True)
+        return float(np.mean(-(zm[np.arange(11), y] - np.log(np.exp(zm).sum(1)))))
+
+    assert loss == pytest.approx(L(p), rel=1e-12)
+    h = 1e-6
+    fd = np.array([(L(p + h * e) - L(p - h * e)) / (2 * h) for e in np.eye(n)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
