@@ -254,11 +254,11 @@ def batched_section(args, torch, d, capi, Hm):
 
     def learned_run():
         feats = np.array([S.features(nm, j["t"], len(D.diagonals), max(abs(o) for o in D.diagonals), D.shape[0],
-                                     tol / 2) for nm, j, D in zip(norms, jobs, Ds)])
+                                     tol / 4) for nm, j, D in zip(norms, jobs, Ds)])
         prop = sel.predict(feats)
         B.set_schedule([j["t"] for j in jobs], [m for m, s in prop], [s for m, s in prop])
         est = B.probe()
-        final = [p if e <= tol / 2 else q for p, q, e in zip(prop, plain, est)]
+        final = [p if e <= tol / 4 else q for p, q, e in zip(prop, plain, est)]
         B.set_schedule([j["t"] for j in jobs], [m for m, s in final], [s for m, s in final])
         ms = B.run(use_graph=True, renormalize=True)
         return prop, final, est, ms
@@ -302,6 +302,17 @@ def batched_section(args, torch, d, capi, Hm):
                                          evolutions_per_s=n_ev / t_b_plain),
         speedup_vs_plain_fp64=t_plain / t_batched,
         reference="plain FP64 fused path at tol 1e-12", reference_seconds=t_ref)
+
+
+def measured_traffic():
+    """HBM bytes per SpMV launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this benchmark's
+    kernel (profiles/spmv_fp64_n2e24_traffic.json, counters scaled by a 2 GiB copy), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "spmv_fp64_n2e24_traffic.json")
+    try:
+        with open(path) as f:
+            return round(float(json.load(f)["traffic_bytes"]))
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -370,7 +381,7 @@ def main():
                    "N": sp["N"], "n_diags": 64, "stored_elements": sp["S"], "compulsory_bytes": sp["bytes"],
                    "spmv_schedule": sp.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": round(sp["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
                          else None),
         "spmv_fp32": sp.get("fp32"),

@@ -41,7 +41,9 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--match", default="spmv_rows_kernel")
     ap.add_argument("--top", type=int, default=6)
+    ap.add_argument("--json", help="also write {kernel, fetch, write, factors, traffic_bytes} to this file")
     a = ap.parse_args()
+    summary = {}
     print("kernel stats (rocprofv3 --kernel-trace --stats), durations in microseconds")
     for name, calls, total, avg, pct in kernel_stats(a.stats)[: a.top]:
         print(f"  {avg:10.1f} us avg  {calls:4d} calls  {pct:5.1f}%  {name[:110]}")
@@ -54,11 +56,18 @@ def main():
             continue
         med = statistics.median(v for v, _ in hit)
         line = f"{label} per dispatch of *{a.match}*: median {med / 1e9:.3f} GB over {len(hit)} dispatches"
+        factor = COPY_BYTES / cal if cal else None
         if cal:
-            factor = COPY_BYTES / cal
             line += (f"; the 2 GiB calibration copy reads {cal / 1e9:.3f} GB, factor {factor:.3f} -> "
                      f"{med * factor / 1e9:.3f} GB")
         print(line)
+        summary[label] = dict(raw_bytes=med, factor=factor, bytes=med * (factor or 1.0))
+    if a.json and "FETCH_SIZE" in summary and "WRITE_SIZE" in summary:
+        import json
+        summary["kernel"] = a.match
+        summary["traffic_bytes"] = summary["FETCH_SIZE"]["bytes"] + summary["WRITE_SIZE"]["bytes"]
+        with open(a.json, "w") as f:
+            json.dump(summary, f, indent=1)
 
 
 if __name__ == "__main__":
