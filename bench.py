@@ -67,9 +67,10 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     # one-time schedule selection for this matrix (outside the timed region, like a library's plan step)
     best = ctypes.c_float()
     capi.check(L.dq_spmv_tune(out, 3, sid, ctypes.byref(best)))
-    cfg = [ctypes.c_int() for _ in range(3)]
+    cfg = [ctypes.c_int() for _ in range(4)]
     capi.check(L.dq_spmv_tuned_config(out, *[ctypes.byref(c) for c in cfg]))
-    tuned = dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value, tune_ms=best.value)
+    tuned = dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value, lds_window=cfg[3].value,
+                 tune_ms=best.value)
     log(f"[bench] spmv tuned schedule: {tuned}")
 
     def step():

@@ -4,6 +4,11 @@
 // of the RPT loads of a wave covers 64 consecutive rows: one aligned line per diagonal and one contiguous window of x.
 // The diagonal loop is uniform across the workgroup; metadata reads are scalar loads. A diagonal that misses the tile
 // (or the column band [col_lo, col_hi)) is skipped by the whole workgroup.
+//
+// LDS x window (HALO > 0): the workgroup first copies x[tile - HALO, tile + TILE + HALO) into LDS (zero outside the
+// matrix columns); every diagonal with |offset| <= HALO then reads its shifted x slice from LDS instead of memory.
+// Neighbouring diagonals (a band, or the +-2^k offsets of a qubit Hamiltonian) share that window, so x crosses the
+// memory hierarchy once per tile instead of once per near diagonal.
 #pragma once
 
 #include "dq_common.hpp"
@@ -37,15 +42,38 @@ __device__ inline T ld_val(const T *p) {
     else return *p;
 }
 
+// Copy x[tile - HALO, tile + TILE + HALO) into LDS (sr, si), zero outside [0, cols), then synchronise the workgroup.
+template <typename T, int RPT, int HALO>
+__device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__restrict__ xr, const T *__restrict__ xi,
+                                      T *__restrict__ sr, T *__restrict__ si) {
+    constexpr int W = 256 * RPT + 2 * HALO;
+    const int64_t ws = tile - HALO;
+    for (int e = threadIdx.x; e < W; e += 256) {
+        const int64_t c = ws + e;
+        T a = T(0), b = T(0);
+        if (c >= 0 && c < cols) {
+            a = xr[c];
+            b = xi[c];
+        }
+        sr[e] = a;
+        si[e] = b;
+    }
+    __syncthreads();
+}
+
 // PIPE = true: each diagonal's 4*RPT loads are issued together (full tiles unconditionally, boundary tiles with
 // clamped addresses), the schedule that keeps the most bytes in flight. PIPE = false: per-row predicated loads, which
 // the compiler issues row after row -- fewer loads in flight per wave but half the registers, so twice the waves;
 // with scattered offsets (no x reuse between diagonals) that variant can fill L2 faster.
-template <typename T, int RPT, bool NT = false, bool PIPE = true>
+//
+// With HALO > 0 the caller has staged the x window in (sr, si) with stage_x_window; diagonals with |offset| <= HALO
+// read x from there (row r of the tile at LDS index r - tile + HALO + offset), the others from memory.
+template <typename T, int RPT, bool NT = false, bool PIPE = true, int HALO = 0>
 __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, const DiagMeta *__restrict__ meta,
                                      const T *__restrict__ are, const T *__restrict__ aim, const T *__restrict__ xr,
                                      const T *__restrict__ xi, int64_t col_lo, int64_t col_hi, Acc<T> (&ar)[RPT],
-                                     Acc<T> (&ai)[RPT]) {
+                                     Acc<T> (&ai)[RPT], const T *__restrict__ sr = nullptr,
+                                     const T *__restrict__ si = nullptr) {
     constexpr int TILE = 256 * RPT;
     for (int k = 0; k < n; ++k) {
         const DiagMeta m = meta[k];
@@ -54,6 +82,40 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
         if (hi <= tile || lo >= tile_end) continue;  // uniform: the diagonal misses this tile
         const T *pr = are + m.rowbase;
         const T *pi = aim + m.rowbase;
+        if (HALO > 0 && m.off >= -HALO && m.off <= HALO) {  // uniform: x from the LDS window
+            const int base = HALO + m.off;
+            T va_r[RPT], va_i[RPT], vx_r[RPT], vx_i[RPT];
+            if (lo <= tile && hi >= tile + TILE) {
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    const int64_t r = tile + threadIdx.x + 256 * j;
+                    va_r[j] = ld_val<T, NT>(pr + r);
+                    va_i[j] = ld_val<T, NT>(pi + r);
+                    vx_r[j] = sr[threadIdx.x + 256 * j + base];
+                    vx_i[j] = si[threadIdx.x + 256 * j + base];
+                }
+            } else {  // boundary tile: clamped rows stay inside the tile, so their LDS index stays in the window
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) {
+                    const int64_t r = tile + threadIdx.x + 256 * j;
+                    const bool in = r >= lo && r < hi;
+                    const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
+                    const T a_r = ld_val<T, NT>(pr + rc), a_i = ld_val<T, NT>(pi + rc);
+                    const int e = static_cast<int>(rc - tile) + base;
+                    const T x_r = sr[e], x_i = si[e];
+                    va_r[j] = in ? a_r : T(0);
+                    va_i[j] = in ? a_i : T(0);
+                    vx_r[j] = in ? x_r : T(0);
+                    vx_i[j] = in ? x_i : T(0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                ar[j].add(va_r[j] * vx_r[j] - va_i[j] * vx_i[j]);
+                ai[j].add(va_r[j] * vx_i[j] + va_i[j] * vx_r[j]);
+            }
+            continue;
+        }
         const T *qr = xr + m.off;
         const T *qi = xi + m.off;
         if constexpr (!PIPE) {

@@ -134,8 +134,9 @@ struct dq_devmat {
     dq::DevBuf re, im;                // values
     dq::DevBuf d_meta;                // DiagMeta[n]
     // SpMV schedule chosen by dq_spmv_tune for this matrix (0 = the library default): rows per lane, non-temporal
-    // matrix loads (-1 = default), load schedule (-1 = default, 0 = per-row predicated, 1 = batched).
-    int tuned_rpt = 0, tuned_nt = -1, tuned_pipe = -1;
+    // matrix loads (-1 = default), load schedule (-1 = default, 0 = per-row predicated, 1 = batched), LDS x window
+    // (-1 = default, 0 = off, 1 = on).
+    int tuned_rpt = 0, tuned_nt = -1, tuned_pipe = -1, tuned_lds = -1;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {
@@ -175,5 +176,11 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);
+
+// Whether the LDS x window is the untuned default for A at rows-per-lane rpt (some off-diagonals within its halo).
+bool lds_window_pays(const dq_devmat &A, int rpt);
+// Whether row-tile kernels over A at rows-per-lane rpt stage the x window in LDS: the process-wide knob if set, else
+// the tuned choice, else lds_window_pays.
+bool lds_window_for(const dq_devmat &A, int rpt);
 
 }  // namespace dq

@@ -10,14 +10,16 @@
 // (Im h * s, -Re h * s) with s = dt/k, and adds term_k to the step's output; the first term reads psi directly and
 // writes out = psi + term_1, so the copies disappear. Per term: H read once, term_{k-1} read, term_k written and the
 // output read and written once (b*S + 4*b*N bytes). The same operation order as the reference per element:
-// row sums in slot order, then the scaling, then the add.
+// row sums in slot order, then the scaling, then the add. The term kernel stages the x window of each row tile in LDS
+// when the matrix's SpMV schedule does (tuned, or some off-diagonals within half a tile): the +-2^k offsets of a qubit
+// Hamiltonian then share one copy of the term vector per tile.
 #include <algorithm>
 
 #include "dia_rows.hpp"
 
 namespace dq {
 
-template <typename T, int RPT>
+template <typename T, int RPT, bool LDS>
 __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
                                                           const T *__restrict__ are, const T *__restrict__ aim,
                                                           const T *__restrict__ tr, const T *__restrict__ ti,
@@ -25,10 +27,14 @@ __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, c
                                                           T *__restrict__ outr, T *__restrict__ outi, double scale,
                                                           int first) {
     constexpr int TILE = 256 * RPT;
+    constexpr int HALO = LDS ? TILE / 2 : 0;
+    constexpr int W = LDS ? TILE + 2 * HALO : 1;
+    __shared__ T sr[W], si[W];
     const int64_t tile = static_cast<int64_t>(blockIdx.x) * TILE;
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
+    if constexpr (LDS) stage_x_window<T, RPT, HALO>(tile, rows, tr, ti, sr, si);
     Acc<T> hr[RPT], hi[RPT];
-    dia_rows_tile<T, RPT, true, true>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi);
+    dia_rows_tile<T, RPT, true, true, HALO>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi, sr, si);
     const T s = static_cast<T>(scale);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
@@ -70,13 +76,21 @@ static void taylor_step(const dq_devmat &H, const T *pr, const T *pi, T *outr, T
     }
     constexpr int RPT = 8;
     const unsigned grid = blocks_for(N, 256 * RPT, 1LL << 31);
+    const bool lds = lds_window_for(H, RPT);
     const T *inr = pr, *ini = pi;
     T *bufr[2] = {t0r, t1r}, *bufi[2] = {t0i, t1i};
     for (int k = 1; k <= m; ++k) {
         T *nr = bufr[k & 1], *ni = bufi[k & 1];
-        hipLaunchKernelGGL((taylor_term_kernel<T, RPT>), dim3(grid), dim3(256), 0, s, N, H.n(),
-                           H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p), static_cast<const T *>(H.im.p),
-                           inr, ini, nr, ni, outr, outi, dt / static_cast<double>(k), k == 1 ? 1 : 0);
+        if (lds)
+            hipLaunchKernelGGL((taylor_term_kernel<T, RPT, true>), dim3(grid), dim3(256), 0, s, N, H.n(),
+                               H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
+                               static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
+                               dt / static_cast<double>(k), k == 1 ? 1 : 0);
+        else
+            hipLaunchKernelGGL((taylor_term_kernel<T, RPT, false>), dim3(grid), dim3(256), 0, s, N, H.n(),
+                               H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
+                               static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
+                               dt / static_cast<double>(k), k == 1 ? 1 : 0);
         DQ_LAUNCH_CHECK();
         inr = nr;
         ini = ni;

@@ -16,6 +16,10 @@
 // 1/n_bands of x, so its x lines stay in the 256 MB Infinity Cache between the diagonals that share them, at the cost
 // of one read and one write of y per extra pass.
 //
+// LDS x window: with lds on, each workgroup stages x[tile - TILE/2, tile + 3 TILE/2) in LDS and the diagonals with
+// |offset| <= TILE/2 read x from there (bands, qubit Hamiltonians); scattered offsets gain nothing from it. The
+// per-matrix tuner picks it with the other schedule choices.
+//
 // FP32: values and vectors are float; each row's diagonal sum is Kahan-compensated in FP32.
 #include <algorithm>
 #include <cstdlib>
@@ -29,16 +33,22 @@ static int g_spmv_rpt = 0;    // rows per lane; 0 = automatic
 static int g_spmv_bands = 0;  // column bands; 0 = automatic
 static int g_spmv_nt = -1;    // non-temporal matrix loads: -1 = automatic, 0 = off, 1 = on
 static int g_spmv_pipe = -1;  // load schedule: -1 = automatic, 0 = per-row predicated, 1 = batched
+static int g_spmv_lds = -1;   // LDS x window: -1 = automatic, 0 = off, 1 = on
 
-template <typename T, int RPT, bool NT, bool PIPE>
-__global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
-                                                        const T *__restrict__ are, const T *__restrict__ aim,
-                                                        const T *__restrict__ xr, const T *__restrict__ xi,
-                                                        T *__restrict__ yr, T *__restrict__ yi, int64_t col_lo,
-                                                        int64_t col_hi, int accumulate) {
+template <typename T, int RPT, bool NT, bool PIPE, bool LDS>
+__global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t cols, int n,
+                                                        const DiagMeta *__restrict__ meta, const T *__restrict__ are,
+                                                        const T *__restrict__ aim, const T *__restrict__ xr,
+                                                        const T *__restrict__ xi, T *__restrict__ yr,
+                                                        T *__restrict__ yi, int64_t col_lo, int64_t col_hi,
+                                                        int accumulate) {
     constexpr int TILE = 256 * RPT;
+    constexpr int HALO = LDS ? TILE / 2 : 0;
+    constexpr int W = LDS ? TILE + 2 * HALO : 1;
+    __shared__ T sr[W], si[W];
     const int64_t tile = static_cast<int64_t>(blockIdx.x) * TILE;
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
+    if constexpr (LDS) stage_x_window<T, RPT, HALO>(tile, cols, xr, xi, sr, si);
     Acc<T> ar[RPT], ai[RPT];
     if (accumulate) {
 #pragma unroll
@@ -47,7 +57,7 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, con
             if (r < rows) { ar[j].add(yr[r]); ai[j].add(yi[r]); }
         }
     }
-    dia_rows_tile<T, RPT, NT, PIPE>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai);
+    dia_rows_tile<T, RPT, NT, PIPE, HALO>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai, sr, si);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int64_t r = tile + threadIdx.x + 256 * j;
@@ -57,19 +67,22 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int n, con
 
 template <typename T>
 static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *yi, int rpt, bool nt, bool pipe,
-                        int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
-    const int64_t rows = A.rows;
+                        bool lds, int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
+    const int64_t rows = A.rows, cols = A.cols;
     const DiagMeta *meta = A.d_meta.as<DiagMeta>();
     const T *are = static_cast<const T *>(A.re.p), *aim = static_cast<const T *>(A.im.p);
-#define DQ_SPMV_LAUNCH(R, NT, PIPE)                                                                           \
-    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE>), dim3(blocks_for(rows, 256 * R, 1LL << 31)), dim3(256), \
-                       0, s, rows, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate)
+#define DQ_SPMV_LAUNCH(R, NT, PIPE, LDS)                                                                         \
+    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE, LDS>), dim3(blocks_for(rows, 256 * R, 1LL << 31)),     \
+                       dim3(256), 0, s, rows, cols, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate)
+#define DQ_SPMV_SCHED(R, LDS)                                         \
+    if (nt && pipe) DQ_SPMV_LAUNCH(R, true, true, LDS);               \
+    else if (nt) DQ_SPMV_LAUNCH(R, true, false, LDS);                 \
+    else if (pipe) DQ_SPMV_LAUNCH(R, false, true, LDS);               \
+    else DQ_SPMV_LAUNCH(R, false, false, LDS);
 #define DQ_SPMV_CASE(R)                                         \
     case R:                                                     \
-        if (nt && pipe) DQ_SPMV_LAUNCH(R, true, true);          \
-        else if (nt) DQ_SPMV_LAUNCH(R, true, false);            \
-        else if (pipe) DQ_SPMV_LAUNCH(R, false, true);          \
-        else DQ_SPMV_LAUNCH(R, false, false);                   \
+        if (lds) { DQ_SPMV_SCHED(R, true) }                     \
+        else { DQ_SPMV_SCHED(R, false) }                        \
         break;
     switch (rpt) {
         DQ_SPMV_CASE(1)
@@ -80,6 +93,7 @@ static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *
         default: throw ArgError("spmv: rows per lane must be 1, 2, 4, 8 or 16");
     }
 #undef DQ_SPMV_LAUNCH
+#undef DQ_SPMV_SCHED
 #undef DQ_SPMV_CASE
     DQ_LAUNCH_CHECK();
 }
@@ -115,6 +129,21 @@ static bool auto_pipe(const dq_devmat &A) {
     return true;
 }
 
+// Untuned default for the LDS window: on when at least two off-diagonals lie within the halo of a tile.
+bool lds_window_pays(const dq_devmat &A, int rpt) {
+    const int halo = 256 * rpt / 2;
+    int near = 0;
+    for (int32_t d : A.offs)
+        if (d != 0 && d >= -halo && d <= halo) ++near;
+    return near >= 2;
+}
+
+bool lds_window_for(const dq_devmat &A, int rpt) {
+    if (g_spmv_lds >= 0) return g_spmv_lds == 1;
+    if (A.tuned_lds >= 0) return A.tuned_lds == 1;
+    return lds_window_pays(A, rpt);
+}
+
 void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, int n_bands,
                  hipStream_t s) {
     if (A.rows == 0) return;
@@ -132,16 +161,16 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     if (n_bands <= 0) n_bands = auto_bands(A);
     if (n_bands > static_cast<int>(A.cols)) n_bands = std::max<int>(1, A.cols);
     const int rpt = auto_rpt(A);
-    const bool nt = auto_nt(A), pipe = auto_pipe(A);
+    const bool nt = auto_nt(A), pipe = auto_pipe(A), lds = lds_window_for(A, rpt);
     const int64_t cols = A.cols;
     for (int b = 0; b < n_bands; ++b) {
         const int64_t c0 = cols * b / n_bands, c1 = cols * (b + 1) / n_bands;
         if (A.precision == DQ_F64)
             launch_rows<double>(A, (const double *)xr, (const double *)xi, (double *)yr, (double *)yi, rpt, nt, pipe,
-                                c0, c1, b > 0, s);
+                                lds, c0, c1, b > 0, s);
         else
-            launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, nt, pipe, c0,
-                               c1, b > 0, s);
+            launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, nt, pipe, lds,
+                               c0, c1, b > 0, s);
     }
 }
 
@@ -167,6 +196,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "spmv_pipelined") {
             if (value < -1 || value > 1) throw ArgError("spmv_pipelined must be -1, 0 or 1");
             g_spmv_pipe = value;
+        } else if (k == "spmv_lds") {
+            if (value < -1 || value > 1) throw ArgError("spmv_lds must be -1, 0 or 1");
+            g_spmv_lds = value;
         } else if (k == "spmv_bands") {
             if (value < 0 || value > 4096) throw ArgError("spmv_bands must be in [0, 4096]");
             g_spmv_bands = value;
@@ -176,8 +208,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
     });
 }
 
-// Time every (load schedule, rows per lane, non-temporal) candidate on scratch vectors and keep the fastest in the
-// matrix handle; later SpMVs with this matrix use it (unless a process-wide knob overrides it).
+// Time every (load schedule, rows per lane, non-temporal, LDS window) candidate on scratch vectors and keep the fastest
+// in the matrix handle; later SpMVs with this matrix use it (unless a process-wide knob overrides it). The LDS window
+// is only tried when some off-diagonal lies within its halo.
 extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms) {
     return guard([&] {
         if (!A) throw ArgError("matrix is NULL");
@@ -192,14 +225,16 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
         DQ_HIP(hipEventCreate(&e0));
         DQ_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        int brpt = 8, bnt = 1, bpipe = 1;
-        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe;
-        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1;
+        int brpt = 8, bnt = 1, bpipe = 1, blds = 0;
+        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe, saved_lds = g_spmv_lds;
+        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1; g_spmv_lds = -1;
         try {
+            for (int lds = 0; lds <= 1; ++lds)
             for (int pipe = 0; pipe <= 1; ++pipe)
                 for (int rpt : {2, 4, 8, 16})
                     for (int nt = 0; nt <= 1; ++nt) {
-                        A->tuned_rpt = rpt; A->tuned_nt = nt; A->tuned_pipe = pipe;
+                        if (lds && !lds_window_pays(*A, rpt) && !lds_window_pays(*A, 16)) continue;
+                        A->tuned_rpt = rpt; A->tuned_nt = nt; A->tuned_pipe = pipe; A->tuned_lds = lds;
                         spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);  // warm
                         DQ_HIP(hipEventRecord(e0, s));
                         for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);
@@ -208,27 +243,29 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
                         float ms = 0.f;
                         DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
                         ms /= repeats;
-                        if (ms < best) { best = ms; brpt = rpt; bnt = nt; bpipe = pipe; }
+                        if (ms < best) { best = ms; brpt = rpt; bnt = nt; bpipe = pipe; blds = lds; }
                     }
         } catch (...) {
-            g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe;
+            g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
             (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
             throw;
         }
-        g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe;
+        g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe;
+        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds;
         if (best_ms) *best_ms = best;
     });
 }
 
-extern "C" int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined) {
+extern "C" int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined,
+                                    int *lds_window) {
     return guard([&] {
         if (!A) throw ArgError("matrix is NULL");
         if (rows_per_lane) *rows_per_lane = auto_rpt(*A);
         if (nontemporal) *nontemporal = auto_nt(*A) ? 1 : 0;
         if (pipelined) *pipelined = auto_pipe(*A) ? 1 : 0;
+        if (lds_window) *lds_window = lds_window_for(*A, auto_rpt(*A)) ? 1 : 0;
     });
 }
 

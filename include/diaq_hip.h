@@ -105,18 +105,20 @@ int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, 
 int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi_dev, void *yr_dev, void *yi_dev,
                           int n_bands, uintptr_t stream);
 
-/* SpMV autotuner: times every (load schedule, rows per lane, non-temporal loads) candidate `repeats` times on
- * scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A. best_ms (may be NULL)
- * receives its time. No reference counterpart. */
+/* SpMV autotuner: times every (load schedule, rows per lane, non-temporal loads, LDS x window) candidate `repeats`
+ * times on scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A (and by the
+ * fused evolution with A). best_ms (may be NULL) receives its time. No reference counterpart. */
 int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 
-/* The SpMV schedule A currently uses (after tuning or the defaults). No reference counterpart. */
-int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined);
+/* The SpMV schedule A currently uses (after tuning or the defaults); any output pointer may be NULL. No reference
+ * counterpart. */
+int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined, int *lds_window);
 
 /* Process-wide tuning knobs for the autotuner and benchmarks: "spmv_rows_per_lane" (0 = automatic, or 1, 2, 4, 8,
  * 16), "spmv_bands" (0 = automatic, or the number of column bands), "spmv_nontemporal" (-1 = automatic, 0, 1:
- * stream the matrix values with non-temporal loads) and "spmv_pipelined" (-1 = automatic, 0 = per-row predicated
- * loads, 1 = batched loads). They override a matrix's tuned schedule. No reference counterpart. */
+ * stream the matrix values with non-temporal loads), "spmv_pipelined" (-1 = automatic, 0 = per-row predicated
+ * loads, 1 = batched loads) and "spmv_lds" (-1 = automatic, 0, 1: stage the x window of each row tile in LDS for the
+ * diagonals within half a tile). They override a matrix's tuned schedule. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------

@@ -169,6 +169,75 @@ def test_spmv_device_capi_streams_and_fp32(d):
             assert rel((yr.double() + 1j * yi.double()).cpu().numpy(), ref) < tol, (prec, bands)
 
 
+@pytest.mark.parametrize("shape,offs", [
+    ((5000, 5000), [-1024, -300, -2, -1, 0, 1, 2, 5, 300, 1024, 3000]),  # near + far diagonals, ragged last tile
+    ((3000, 4500), [-2999, -129, -64, 0, 64, 128, 129, 4499]),           # wide, near diagonals on both halo edges
+    ((4500, 3000), [-4499, -2048, -7, 0, 7, 2047, 2999]),                # tall
+    ((100, 100), [-99, -3, -1, 0, 1, 3, 99]),                            # smaller than one tile: window off both ends
+    ((9000, 9000), list(range(-40, 41, 3))),                              # a band
+])
+def test_spmv_every_schedule(d, shape, offs):
+    """Every (rows per lane, non-temporal, load schedule, LDS x window) combination of the SpMV kernel, forced through
+    the process-wide knobs, against the oracle in FP64 and FP32 (Kahan-compensated sums)."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    rng = np.random.default_rng(11)
+    vals = [rng.standard_normal(O.diagonal_length(o, *shape)) + 1j * rng.standard_normal(O.diagonal_length(o, *shape))
+            for o in offs]
+    M = O.Dia(shape[0], shape[1], list(offs), vals)
+    x = rng.standard_normal(shape[1]) + 1j * rng.standard_normal(shape[1])
+    ref = O.spmv(M, x)
+    H = dia_of(d, shape[0], shape[1], offs, vals)
+    try:
+        for prec, tol, dt in (("fp64", TOL, torch.float64), ("fp32", 2e-6, torch.float32)):
+            D = d.to_device(H, precision=prec)
+            xr = torch.as_tensor(x.real, device="cuda").to(dt).contiguous()
+            xi = torch.as_tensor(x.imag, device="cuda").to(dt).contiguous()
+            for rpt in (1, 2, 4, 8, 16):
+                for nt in (0, 1):
+                    for pipe in (0, 1):
+                        for lds in (0, 1):
+                            for key, v in (("spmv_rows_per_lane", rpt), ("spmv_nontemporal", nt),
+                                           ("spmv_pipelined", pipe), ("spmv_lds", lds)):
+                                capi.check(L.dq_set_tuning(key.encode(), v))
+                            yr = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
+                            yi = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
+                            capi.check(L.dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(),
+                                                        yi.data_ptr(), 0))
+                            torch.cuda.synchronize()
+                            got = (yr.double() + 1j * yi.double()).cpu().numpy()
+                            assert rel(got, ref) < tol, (prec, rpt, nt, pipe, lds)
+    finally:
+        for key, v in (("spmv_rows_per_lane", 0), ("spmv_nontemporal", -1), ("spmv_pipelined", -1), ("spmv_lds", -1)):
+            capi.check(L.dq_set_tuning(key.encode(), v))
+
+
+def test_spmv_tuner_keeps_results(d):
+    """dq_spmv_tune picks a schedule per matrix (the LDS window only where near diagonals exist); the tuned SpMV gives
+    the oracle's values."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    rng = np.random.default_rng(12)
+    for N, offs, lds_possible in ((1 << 16, list(range(-8, 9)), True), (1 << 16, Hm.random_offsets(1 << 16, 9, 4),
+                                                                         False)):
+        vals = [rng.standard_normal(O.diagonal_length(o, N, N)) + 1j * rng.standard_normal(O.diagonal_length(o, N, N))
+                for o in offs]
+        M = O.Dia(N, N, list(offs), vals)
+        x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        D = d.to_device(dia_of(d, N, N, offs, vals))
+        best = ctypes.c_float()
+        capi.check(L.dq_spmv_tune(D.handle, 2, 0, ctypes.byref(best)))
+        cfg = [ctypes.c_int(-5) for _ in range(4)]
+        capi.check(L.dq_spmv_tuned_config(D.handle, *[ctypes.byref(c) for c in cfg]))
+        assert cfg[0].value in (2, 4, 8, 16) and cfg[1].value in (0, 1) and cfg[2].value in (0, 1)
+        assert cfg[3].value in ((0, 1) if lds_possible else (0,))
+        assert best.value > 0
+        got = d.to_numpy_vector(d.spMV(D, d.from_numpy_vector(x)))
+        assert rel(got, O.spmv(M, x)) < TOL
+
+
 def test_spmv_full_size_property_2p24(d):
     """N = 2^24 with 64 random offsets (the SpMV benchmark shape) -- values from closed-form functions evaluated on
     both sides; rows are checked on a sample against the oracle rule, and linearity on the whole vector."""
@@ -363,14 +432,20 @@ def test_expm_positive_and_single_diagonal(d):
 # Fused evolution
 def test_hamsim_vs_oracle(d):
     import scipy.sparse.linalg
+    from diaq_amd import capi
     rows, cols, offs, vals = Hm.xxz(10, seed=1)
     M = O.Dia(rows, cols, offs, vals)
     psi = Hm.product_state(10, 2)
     H = dia_of(d, rows, cols, offs, vals)
-    for t, r, m in ((0.5, 5, 12), (1.0, 3, 30)):
-        got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), t, r, m))
-        ref = O.hamiltonian_simulation(M, psi, t, r, m)
-        assert rel(got, ref) < TOL
+    try:
+        for lds in (0, 1, -1):  # term kernel without / with the LDS x window, then the automatic choice
+            capi.check(capi.lib().dq_set_tuning(b"spmv_lds", lds))
+            for t, r, m in ((0.5, 5, 12), (1.0, 3, 30)):
+                got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), t, r, m))
+                ref = O.hamiltonian_simulation(M, psi, t, r, m)
+                assert rel(got, ref) < TOL, (lds, t, r, m)
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"spmv_lds", -1))
     exact = scipy.sparse.linalg.expm_multiply(-1j * 1.0 * O.to_dense(M), psi)
     got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 10, 20))
     assert np.linalg.norm(got - exact) < 1e-10
@@ -390,6 +465,19 @@ def test_hamsim_device_in_place(d):
     torch.cuda.synchronize()
     got = (pr + 1j * pi).cpu().numpy()
     assert rel(got, O.hamiltonian_simulation(M, psi, 0.6, 3, 15)) < TOL
+    # FP32 form with the LDS window forced on and off: within the FP32 bound of the FP64 result
+    from diaq_amd import capi
+    D32 = d.to_device(dia_of(d, rows, cols, offs, vals), precision="fp32")
+    try:
+        for lds in (0, 1):
+            capi.check(capi.lib().dq_set_tuning(b"spmv_lds", lds))
+            qr = torch.as_tensor(psi.real, device="cuda").float().contiguous()
+            qi = torch.as_tensor(psi.imag, device="cuda").float().contiguous()
+            d.hamsim_device(D32, qr.data_ptr(), qi.data_ptr(), 0.6, 3, 15)
+            torch.cuda.synchronize()
+            assert rel((qr.double() + 1j * qi.double()).cpu().numpy(), got) < 5e-6, lds
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"spmv_lds", -1))
 
 
 # ---------------------------------------------------------------------------------------------------------------------
