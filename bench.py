@@ -173,19 +173,19 @@ def norm_rule_m(norm1_dt: float, tol: float, m_max: int = 100) -> int:
     return m_max
 
 
-def evolution_section(args, torch, d, capi, Hm):
+def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True):
     n = 20
     rows, cols, offs, vals = Hm.xxz(n, 1.0, 0.8, seed=0)
     H = d.to_device(d.diaQMatrix.from_diagonals([rows, cols], offs, vals))
     psi0 = Hm.product_state(n, seed=0)
-    steps, dt = 100, 0.05
+    dt = 0.05
     norm1 = d.estimate_1norm(H)
     tol_step = 1e-6 / steps  # per-step truncation budget for a 1e-6 total
     m_plain = norm_rule_m(norm1 * dt, tol_step)
     out = dict(workload=f"XXZ-{n} (Jxy=1, Jz=0.8, h_i~U[-1,1] seed 0), {steps} steps of dt={dt}, product state seed 0",
                norm1=norm1, m_plain=m_plain, tol_step=tol_step)
     sid = torch.cuda.current_stream().cuda_stream
-    for label, m in (("plain_fp64", m_plain), ("reference_m100", 100)):
+    for label, m in (("plain_fp64", m_plain), ("reference_m100", 100))[: 2 if m100 else 1]:
         pr = torch.as_tensor(psi0.real, device="cuda").contiguous()
         pi = torch.as_tensor(psi0.imag, device="cuda").contiguous()
         torch.cuda.synchronize()
@@ -203,7 +203,135 @@ def evolution_section(args, torch, d, capi, Hm):
     S = H.stored
     out["bytes_per_term"] = 16 * S + 64 * rows
     out["plain_fp64"]["gbs"] = out["bytes_per_term"] * m_plain * steps / out["plain_fp64"]["seconds"] / 1e9
+    if not fp32:
+        return out
+    # precision form: the same evolution with FP32 values and vectors (Kahan-compensated row sums)
+    H32 = H.astype("fp32")
+    pr = torch.as_tensor(psi0.real, device="cuda").float().contiguous()
+    pi = torch.as_tensor(psi0.imag, device="cuda").float().contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.hamsim_device(H32, pr.data_ptr(), pi.data_ptr(), steps * dt, steps, m_plain, sid)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    st = torch.complex(pr.double(), pi.double())
+    b32 = 8 * S + 32 * rows
+    out["fp32"] = dict(m=m_plain, seconds=t, evolutions_per_s=1.0 / t, gbs=b32 * m_plain * steps / t / 1e9,
+                       l2_err_vs_plain_fp64=float(torch.linalg.vector_norm(st - ref_state)),
+                       speedup_vs_plain_fp64=out["plain_fp64"]["seconds"] / t)
+    log(f"[bench] evolution fp32: m={m_plain} {t:.3f} s/evolution, error vs FP64 {out['fp32']['l2_err_vs_plain_fp64']:.1e}")
     return out
+
+
+def structured_spmv_section(args, torch, d, capi, Hm):
+    """SpMV on a matrix of the headline's size with the offsets of a 24-qubit transverse-field Ising Hamiltonian
+    (0 and +-2^k, k = 0..23; complex-normal values): what the LDS x window gains where diagonals share x."""
+    import ctypes
+    n = 24
+    N = 1 << n
+    offs = sorted([0] + [1 << k for k in range(n)] + [-(1 << k) for k in range(n)])
+    L = capi.lib()
+    A = ctypes.c_void_p()
+    o32 = np.asarray(offs, np.int32)
+    capi.check(L.dq_devmat_zeros(N, N, len(offs), capi.ptr(o32, ctypes.c_int32), capi.DQ_F64, ctypes.byref(A)))
+    try:
+        S = Hm.stored_elements(N, N, offs)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(2)
+        re = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+        im = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+        capi.check(L.dq_devmat_set_packed_device(A, re.data_ptr(), im.data_ptr(), 0))
+        del re, im
+        xr = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+        xi = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        stream = torch.cuda.current_stream()
+        sid = stream.cuda_stream
+        best = ctypes.c_float()
+        capi.check(L.dq_spmv_tune(A, 3, sid, ctypes.byref(best)))
+        cfg = [ctypes.c_int() for _ in range(4)]
+        capi.check(L.dq_spmv_tuned_config(A, *[ctypes.byref(c) for c in cfg]))
+        for _ in range(args.warmup):
+            capi.check(L.dq_spmv_device(A, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            capi.check(L.dq_spmv_device(A, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        nbytes = 16 * S + 32 * N
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        log(f"[bench] structured spmv (TFIM-24 offsets): {ms:.3f} ms  {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.3f} of peak)")
+        return dict(workload="DIA SpMV, N=2^24, offsets 0 and +-2^k (k=0..23, a 24-qubit TFIM's), complex-normal "
+                             "values (torch seed 2), complex128", N=N, n_diags=len(offs), stored_elements=S,
+                    compulsory_bytes=nbytes, ms=ms, gbs=gbs, frac=gbs / HBM_PEAK_GBS,
+                    schedule=dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value,
+                                  lds_window=cfg[3].value))
+    finally:
+        L.dq_devmat_destroy(A)
+        torch.cuda.synchronize()
+
+
+def spgemm_section(args, torch, d, capi, Hm, precisions=("fp64", "fp32")):
+    """SpGEMM H*H for the 20-qubit TFIM (h = 0.7, J = 1) on device matrices, FP64 and FP32: time per product (dry-run
+    + kernel, the result allocated by the call), compulsory bytes b*(S_A + S_B + S_C), the output offset count and the
+    FP32 result's relative Frobenius error against FP64."""
+    n = 20
+    r, c, offs, vals = Hm.tfim(n, 1.0, 0.7)
+    H = d.to_device(d.diaQMatrix.from_diagonals([r, c], offs, vals))
+    out = dict(workload=f"TFIM-{n} H*H (J=1, h=0.7), DeviceMatrix product (no prune)")
+    prods = {}
+    for prec in precisions:
+        A = H if prec == "fp64" else H.astype("fp32")
+        C = d.product(A, A)  # warmup (and the result kept for the error)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(3):
+            del C
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            C = d.product(A, A)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        b = 16 if prec == "fp64" else 8
+        nbytes = b * (2 * A.stored + C.stored)
+        out[prec] = dict(seconds=t, n_offsets=len(C.diagonals), stored_result=C.stored, compulsory_bytes=nbytes,
+                         gbs=nbytes / t / 1e9)
+        log(f"[bench] spgemm {prec}: {t * 1e3:.2f} ms  {nbytes / t / 1e9:.1f} GB/s  {len(C.diagonals)} offsets")
+        prods[prec] = C
+    if "fp32" not in prods:
+        return out
+    out["offsets_identical"] = list(prods["fp64"].diagonals) == list(prods["fp32"].diagonals)
+    out["fp32_rel_frobenius_err"] = d.norm_diff(prods["fp32"].astype("fp64"), prods["fp64"]) / d.norm(prods["fp64"])
+    log(f"[bench] spgemm fp32 vs fp64: rel Frobenius {out['fp32_rel_frobenius_err']:.2e}, offsets identical "
+        f"{out['offsets_identical']}")
+    return out
+
+
+def expm_section(args, torch, d, capi, Hm):
+    """expm(-i H t) at t = 0.1 for the 12-qubit TFIM (J = 1, h = 0.7) by Taylor with scaling and squaring on the GPU,
+    against scipy's dense expm."""
+    import scipy.linalg
+    r, c, offs, vals = Hm.tfim(12, 1.0, 0.7)
+    A = d.diaQMatrix.from_diagonals([r, c], offs, vals)
+    U = d.expm_negative(A, 0.1)  # warmup
+    t0 = time.perf_counter()
+    U = d.expm_negative(A, 0.1)
+    t = time.perf_counter() - t0
+    dense = np.zeros((r, c), complex)
+    for o, v in zip(offs, vals):
+        k = np.arange(len(v))
+        rows_, cols_ = (k, k + o) if o >= 0 else (k - o, k)
+        dense[rows_, cols_] = v
+    ref = scipy.linalg.expm(-1j * 0.1 * dense)
+    got = d.to_numpy(U)
+    err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    log(f"[bench] expm_negative TFIM-12 t=0.1: {t:.3f} s, {U.num_diags()} diagonals, max rel error vs scipy {err:.1e}")
+    return dict(workload="expm(-i H t), TFIM-12 (J=1, h=0.7), t=0.1, defaults max_it 100, cutoff 1, tol 1e-12",
+                seconds=t, n_diagonals=U.num_diags(), max_rel_err_vs_scipy_expm=err)
 
 
 def batched_section(args, torch, d, capi, Hm):
@@ -324,9 +452,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="profiling run: the FP64 SpMV steps only, then one plain copy of 2 GiB (a kernel of known "
-                         "size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--profile", nargs="?", const="spmv", default=None,
+                    choices=["spmv", "spmv_structured", "spgemm", "evolution"],
+                    help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
+                         "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -341,12 +470,23 @@ def main():
     torch.cuda.set_device(0)
     if args.profile:
         args.no_cpu = True
-        sp = spmv_section(args, torch, d, capi, Hm, fp32=False)
+        if args.profile == "spmv":
+            sp = spmv_section(args, torch, d, capi, Hm, fp32=False)
+            log(f"[bench] profile: spmv {sp['ms']:.3f} ms")
+        elif args.profile == "spmv_structured":
+            sp = structured_spmv_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: structured spmv {sp['ms']:.3f} ms")
+        elif args.profile == "spgemm":
+            sg = spgemm_section(args, torch, d, capi, Hm, precisions=("fp64",))
+            log(f"[bench] profile: spgemm {sg['fp64']['seconds'] * 1e3:.2f} ms")
+        else:
+            ev = evolution_section(args, torch, d, capi, Hm, steps=10, fp32=False, m100=False)
+            log(f"[bench] profile: evolution {ev['plain_fp64']['seconds']:.3f} s")
         src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")  # 2 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"[bench] profile: spmv {sp['ms']:.3f} ms; calibration copy of {src.numel() * 8} bytes done")
+        log(f"[bench] profile: calibration copy of {src.numel() * 8} bytes done")
         return
     sp = spmv_section(args, torch, d, capi, Hm)
     evo = None
@@ -356,6 +496,15 @@ def main():
         except Exception as e:
             evo = dict(error=str(e))
             log(f"[bench] evolution failed: {e}")
+    extra = {}
+    if not args.no_evolution:
+        for key, fn in (("spmv_structured", structured_spmv_section), ("spgemm", spgemm_section),
+                        ("expm", expm_section)):
+            try:
+                extra[key] = fn(args, torch, d, capi, Hm)
+            except Exception as e:
+                extra[key] = dict(error=str(e))
+                log(f"[bench] {key} failed: {e}")
     bat = None
     if not args.no_batched:
         try:
@@ -387,7 +536,10 @@ def main():
                          else None),
         "spmv_fp32": sp.get("fp32"),
         "cpu_detail": cpu or None,
+        "spmv_structured": extra.get("spmv_structured"),
         "evolution": evo,
+        "spgemm": extra.get("spgemm"),
+        "expm": extra.get("expm"),
         "batched_evolution": bat,
     }
     print(json.dumps(line), flush=True)

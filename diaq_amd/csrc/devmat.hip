@@ -37,7 +37,7 @@ void dev_free(void *p) {
 // Layout
 // ------------------------------------------------------------------------------------------------------------------
 std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32_t *offsets, int n, int precision,
-                                       hipStream_t stream) {
+                                       hipStream_t stream, bool zero_fill) {
     if (n < 0) throw ArgError("negative number of diagonals");
     if (precision != DQ_F64 && precision != DQ_F32) throw ArgError("precision must be DQ_F64 or DQ_F32");
     if (n > 0 && !offsets) throw ArgError("offsets is NULL");
@@ -68,7 +68,7 @@ std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32
     const size_t vb = static_cast<size_t>(M->total) * elem_bytes(precision);
     M->re = DevBuf(vb);
     M->im = DevBuf(vb);
-    if (vb) {
+    if (vb && zero_fill) {
         DQ_HIP(hipMemsetAsync(M->re.p, 0, vb, stream));
         DQ_HIP(hipMemsetAsync(M->im.p, 0, vb, stream));
     }

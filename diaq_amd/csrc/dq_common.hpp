@@ -148,9 +148,10 @@ struct dq_devmat {
 
 namespace dq {
 
-// Compute the layout of (rows, cols, offsets) and allocate zeroed values and device metadata (stream-ordered).
+// Compute the layout of (rows, cols, offsets) and allocate the values (zero-filled unless zero_fill is false, for a
+// caller that writes every element) and the device metadata (stream-ordered).
 std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32_t *offsets, int n, int precision,
-                                       hipStream_t stream);
+                                       hipStream_t stream, bool zero_fill = true);
 
 // Launch geometry helpers.
 inline unsigned blocks_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
@@ -170,6 +171,9 @@ double frobenius(const dq_devmat &A, hipStream_t s);
 double estimate_1norm(const dq_devmat &A, hipStream_t s);
 std::unique_ptr<dq_devmat> prune(const dq_devmat &A, hipStream_t s);
 std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
+
+// Product tuning keys ("product_interleave", "product_nt_store"); false if the key is not one of them.
+bool product_set_tuning(const std::string &key, int value);
 
 // C = (A B) / divisor with the structural dry-run (product.hip).
 std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);

@@ -5,14 +5,20 @@
 //
 // Dry-run (host, integer only): every pair (slot a, slot b) with dC = dA + dB in [-(rows-1), cols-1] contributes to
 // result diagonal dC; result offsets ascend and each dC keeps its pairs in A-slot order (then B-slot order), so the
-// sums run in the reference's order.
+// sums run in the reference's order. The pairs are bucketed by dC with a counting sort (two passes over the slot
+// pairs, in slot order, so each bucket keeps the enumeration order); offset ranges too wide for a count array fall
+// back to a stable sort.
 //
 // Kernel: in row space the pair (a, b) adds A_a[i] * B_b[i + dA] to C_c[i] (row i of A, row i + dA of B), where a
 // diagonal's element for row r sits at rowbase + r; rows outside a diagonal's range contribute nothing, exactly the
-// reference's position bounds. The work is cut into items of (result diagonal, chunk of 256*RPT rows) -- fixed-size
-// items, so result diagonals of length 1 and of length N balance -- and items are ordered by row chunk so the
-// workgroups of one row chunk, which read the same rows of A's diagonals, run together; the item index is remapped
-// so those workgroups land on the same XCD (workgroup b runs on XCD b mod 8) and share its L2.
+// reference's position bounds. The work is cut into items of (row chunk of 256*RPT rows, result diagonal) --
+// fixed-size items, so result diagonals of length 1 and of length N balance. Items are ordered by row chunk: the
+// result diagonals meeting a chunk form a contiguous range of the ascending result offsets, so the host only sizes
+// each chunk's range and a device kernel writes the item list. The item index is remapped so the workgroups of one
+// row chunk, which read the same rows of A's diagonals, land on the same XCD (workgroup b runs on XCD b mod 8) and
+// share its L2. A workgroup walks its diagonal's pairs in batches of 256: the lanes load the batch's pair metadata
+// in parallel and leave each pair's row range and base pointers in LDS, then the pair loop reads them from there
+// (no chain of dependent metadata loads per pair).
 #include <algorithm>
 #include <numeric>
 
@@ -28,12 +34,48 @@ struct DryRun {
 
 static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &oa, const std::vector<int32_t> &ob) {
     const int64_t lo = -static_cast<int64_t>(rows) + 1, hi = static_cast<int64_t>(cols) - 1;
+    DryRun r;
+    const int64_t range = hi - lo + 1;
+    const int64_t n_ab = static_cast<int64_t>(oa.size()) * static_cast<int64_t>(ob.size());
+    if (range <= 0 || n_ab == 0) {
+        r.pair_start.push_back(0);
+        return r;
+    }
+    if (range <= std::max<int64_t>(int64_t(1) << 24, 4 * n_ab)) {  // counting sort by dC
+        std::vector<int64_t> cnt(static_cast<size_t>(range), 0);
+        for (size_t a = 0; a < oa.size(); ++a)
+            for (size_t b = 0; b < ob.size(); ++b) {
+                const int64_t dc = static_cast<int64_t>(oa[a]) + ob[b];
+                if (dc >= lo && dc <= hi) ++cnt[static_cast<size_t>(dc - lo)];
+            }
+        int64_t total = 0;
+        for (int64_t i = 0; i < range; ++i) {
+            if (!cnt[i]) continue;
+            r.res.push_back(static_cast<int32_t>(i + lo));
+            r.pair_start.push_back(total);
+            const int64_t c = cnt[i];
+            cnt[i] = total;  // becomes the bucket's fill position
+            total += c;
+        }
+        r.pair_start.push_back(total);
+        r.sa.resize(static_cast<size_t>(total));
+        r.sb.resize(static_cast<size_t>(total));
+        for (size_t a = 0; a < oa.size(); ++a)
+            for (size_t b = 0; b < ob.size(); ++b) {
+                const int64_t dc = static_cast<int64_t>(oa[a]) + ob[b];
+                if (dc < lo || dc > hi) continue;
+                const int64_t at = cnt[static_cast<size_t>(dc - lo)]++;
+                r.sa[at] = static_cast<int32_t>(a);
+                r.sb[at] = static_cast<int32_t>(b);
+            }
+        return r;
+    }
     struct P {
         int64_t dc;
         int32_t a, b;
     };
     std::vector<P> pairs;
-    pairs.reserve(oa.size() * ob.size());
+    pairs.reserve(static_cast<size_t>(n_ab));
     for (size_t a = 0; a < oa.size(); ++a)
         for (size_t b = 0; b < ob.size(); ++b) {
             const int64_t dc = static_cast<int64_t>(oa[a]) + ob[b];
@@ -42,7 +84,6 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
         }
     // stable by construction order (a, then b): sort by dc only
     std::stable_sort(pairs.begin(), pairs.end(), [](const P &x, const P &y) { return x.dc < y.dc; });
-    DryRun r;
     r.sa.reserve(pairs.size());
     r.sb.reserve(pairs.size());
     for (size_t p = 0; p < pairs.size(); ++p) {
@@ -57,80 +98,134 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
     return r;
 }
 
-struct WorkItem {
-    int64_t r0;     // first row
-    int32_t c;      // result slot
-    int32_t nrows;  // rows in this item
-};
-
-constexpr int PROD_RPT = 4;
+constexpr int PROD_RPT = 8;
 constexpr int PROD_CHUNK = 256 * PROD_RPT;
+constexpr int PROD_BATCH = 256;
 
+// Process-wide knobs (dq_set_tuning): chunk order across XCDs and non-temporal stores of C.
+static int g_prod_interleave = 1;
+static int g_prod_nt_store = 1;
+
+// Item list: the i-th chunk in processing order (row chunk q_of[i]) owns items [first[i], first[i + 1]), result
+// diagonals c_lo[i], c_lo[i] + 1, ...
+__global__ void product_items_kernel(int64_t n_chunks, const int64_t *__restrict__ first,
+                                     const int32_t *__restrict__ c_lo, const int32_t *__restrict__ q_of,
+                                     int2 *__restrict__ items) {
+    for (int64_t i = blockIdx.x; i < n_chunks; i += gridDim.x) {
+        const int64_t f = first[i], n = first[i + 1] - f;
+        const int32_t c0 = c_lo[i], q = q_of[i];
+        for (int64_t j = threadIdx.x; j < n; j += blockDim.x) items[f + j] = make_int2(q, c0 + static_cast<int>(j));
+    }
+}
+
+// Zero the storage outside every diagonal's rows: the alignment gaps between diagonals and the tail.
 template <typename T>
-__global__ __launch_bounds__(256) void product_kernel(const WorkItem *__restrict__ work, int64_t n_work,
-                                                      int64_t per_xcd, const int64_t *__restrict__ pair_start,
+__global__ void zero_gaps_kernel(int n, const DiagMeta *__restrict__ meta, int64_t total, T *__restrict__ re,
+                                 T *__restrict__ im) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n) return;
+    const int64_t b = k == 0 ? 0 : meta[k - 1].rowbase + meta[k - 1].rhi;
+    const int64_t e = k == n ? total : meta[k].rowbase + meta[k].rlo;
+    for (int64_t i = b; i < e; ++i) {
+        re[i] = T(0);
+        im[i] = T(0);
+    }
+}
+
+__device__ inline int64_t uniform64(int64_t v) {
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint64_t>(v) & 0xffffffffu));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo));
+}
+
+template <typename T, bool NTS>
+__global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ items, const int64_t *__restrict__ xcd_first,
+                                                      const int64_t *__restrict__ pair_start,
                                                       const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                                                       const DiagMeta *__restrict__ ma, const DiagMeta *__restrict__ mb,
                                                       const DiagMeta *__restrict__ mc, const T *__restrict__ are,
                                                       const T *__restrict__ aim, const T *__restrict__ bre,
                                                       const T *__restrict__ bim, T *__restrict__ cre,
                                                       T *__restrict__ cim, double divisor) {
-    // XCD-aware remap: the workgroups of XCD x (x = b mod 8) take the contiguous item range [x*per_xcd, ...).
+    __shared__ int64_t s_abase[PROD_BATCH], s_bbase[PROD_BATCH];
+    __shared__ int32_t s_lo[PROD_BATCH], s_hi[PROD_BATCH];
+    // XCD-aware remap: the workgroups of XCD x (x = b mod 8) take the item range [xcd_first[x], xcd_first[x + 1]).
     const int64_t b = blockIdx.x;
-    const int64_t w = (b & 7) * per_xcd + (b >> 3);
-    if (w >= n_work) return;
-    const WorkItem it = work[w];
-    const DiagMeta m_c = mc[it.c];
-    const int64_t p0 = pair_start[it.c], p1 = pair_start[it.c + 1];
+    const int64_t w = xcd_first[b & 7] + (b >> 3);
+    if (w >= xcd_first[(b & 7) + 1]) return;  // uniform
+    const int2 it = items[w];
+    const DiagMeta m_c = mc[it.y];
+    const int64_t q0 = static_cast<int64_t>(it.x) * PROD_CHUNK;
+    const int64_t r0 = max(m_c.rlo, q0), r1 = min(m_c.rhi, q0 + PROD_CHUNK);
+    const int64_t p0 = pair_start[it.y], p1 = pair_start[it.y + 1];
     T acc_r[PROD_RPT], acc_i[PROD_RPT];
 #pragma unroll
     for (int j = 0; j < PROD_RPT; ++j) { acc_r[j] = T(0); acc_i[j] = T(0); }
-    const int64_t r_end = it.r0 + it.nrows;
-    for (int64_t p = p0; p < p1; ++p) {
-        const DiagMeta ta = ma[pa[p]];
-        const DiagMeta tb = mb[pb[p]];
-        // rows r of this item with r on A's diagonal and r + dA on B's: [lo, hi)
-        const int64_t lo = max(max(ta.rlo, tb.rlo - ta.off), it.r0);
-        const int64_t hi = min(min(ta.rhi, tb.rhi - ta.off), r_end);
-        if (hi <= lo) continue;  // uniform
-        const T *apr = are + ta.rowbase, *api = aim + ta.rowbase;
-        const T *bpr = bre + tb.rowbase + ta.off, *bpi = bim + tb.rowbase + ta.off;
-        T va_r[PROD_RPT], va_i[PROD_RPT], vb_r[PROD_RPT], vb_i[PROD_RPT];
-        if (lo == it.r0 && hi == it.r0 + PROD_CHUNK) {  // uniform: the whole item, no per-row checks
+    for (int64_t pbat = p0; pbat < p1; pbat += PROD_BATCH) {
+        const int nb = static_cast<int>(min<int64_t>(PROD_BATCH, p1 - pbat));
+        if (static_cast<int>(threadIdx.x) < nb) {
+            const int64_t p = pbat + threadIdx.x;
+            const DiagMeta ta = ma[pa[p]];
+            const DiagMeta tb = mb[pb[p]];
+            // rows r of this item with r on A's diagonal and r + dA on B's: [lo, hi), as offsets from q0
+            const int64_t lo = max(max(ta.rlo, tb.rlo - ta.off), r0);
+            const int64_t hi = min(min(ta.rhi, tb.rhi - ta.off), r1);
+            s_abase[threadIdx.x] = ta.rowbase;
+            s_bbase[threadIdx.x] = tb.rowbase + ta.off;
+            s_lo[threadIdx.x] = hi > lo ? static_cast<int32_t>(lo - q0) : 0;
+            s_hi[threadIdx.x] = hi > lo ? static_cast<int32_t>(hi - q0) : 0;
+        }
+        __syncthreads();
+        for (int k = 0; k < nb; ++k) {
+            const int64_t lo = q0 + __builtin_amdgcn_readfirstlane(s_lo[k]);
+            const int64_t hi = q0 + __builtin_amdgcn_readfirstlane(s_hi[k]);
+            if (hi <= lo) continue;  // uniform
+            const int64_t abase = uniform64(s_abase[k]), bbase = uniform64(s_bbase[k]);
+            const T *apr = are + abase, *api = aim + abase;
+            const T *bpr = bre + bbase, *bpi = bim + bbase;
+            T va_r[PROD_RPT], va_i[PROD_RPT], vb_r[PROD_RPT], vb_i[PROD_RPT];
+            if (lo == q0 && hi == q0 + PROD_CHUNK) {  // uniform: the whole chunk, no per-row checks
 #pragma unroll
-            for (int j = 0; j < PROD_RPT; ++j) {
-                const int64_t r = it.r0 + threadIdx.x + 256 * j;
-                va_r[j] = apr[r]; va_i[j] = api[r]; vb_r[j] = bpr[r]; vb_i[j] = bpi[r];
+                for (int j = 0; j < PROD_RPT; ++j) {
+                    const int64_t r = q0 + threadIdx.x + 256 * j;
+                    va_r[j] = apr[r]; va_i[j] = api[r]; vb_r[j] = bpr[r]; vb_i[j] = bpi[r];
+                }
+            } else {  // clamp to a valid row, zero both operands of the rows outside [lo, hi)
+#pragma unroll
+                for (int j = 0; j < PROD_RPT; ++j) {
+                    const int64_t r = q0 + threadIdx.x + 256 * j;
+                    const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
+                    va_r[j] = apr[rc]; va_i[j] = api[rc]; vb_r[j] = bpr[rc]; vb_i[j] = bpi[rc];
+                }
+#pragma unroll
+                for (int j = 0; j < PROD_RPT; ++j) {
+                    const int64_t r = q0 + threadIdx.x + 256 * j;
+                    if (r < lo || r >= hi) { va_r[j] = T(0); va_i[j] = T(0); vb_r[j] = T(0); vb_i[j] = T(0); }
+                }
             }
-        } else {  // clamp to a valid row, zero both operands of the rows outside [lo, hi)
 #pragma unroll
             for (int j = 0; j < PROD_RPT; ++j) {
-                const int64_t r = it.r0 + threadIdx.x + 256 * j;
-                const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
-                va_r[j] = apr[rc]; va_i[j] = api[rc]; vb_r[j] = bpr[rc]; vb_i[j] = bpi[rc];
-            }
-#pragma unroll
-            for (int j = 0; j < PROD_RPT; ++j) {
-                const int64_t r = it.r0 + threadIdx.x + 256 * j;
-                if (r < lo || r >= hi) { va_r[j] = T(0); va_i[j] = T(0); vb_r[j] = T(0); vb_i[j] = T(0); }
+                acc_r[j] += va_r[j] * vb_r[j] - va_i[j] * vb_i[j];
+                acc_i[j] += va_r[j] * vb_i[j] + va_i[j] * vb_r[j];
             }
         }
-#pragma unroll
-        for (int j = 0; j < PROD_RPT; ++j) {
-            acc_r[j] += va_r[j] * vb_r[j] - va_i[j] * vb_i[j];
-            acc_i[j] += va_r[j] * vb_i[j] + va_i[j] * vb_r[j];
-        }
+        __syncthreads();  // the next batch overwrites the staged pairs
     }
 #pragma unroll
     for (int j = 0; j < PROD_RPT; ++j) {
-        const int64_t r = it.r0 + threadIdx.x + 256 * j;
-        if (r < it.r0 + it.nrows) {
+        const int64_t r = q0 + threadIdx.x + 256 * j;
+        if (r >= r0 && r < r1) {
+            T vr = acc_r[j], vi = acc_i[j];
             if (divisor != 1.0) {
-                cre[m_c.rowbase + r] = static_cast<T>(acc_r[j] / divisor);
-                cim[m_c.rowbase + r] = static_cast<T>(acc_i[j] / divisor);
+                vr = static_cast<T>(acc_r[j] / divisor);
+                vi = static_cast<T>(acc_i[j] / divisor);
+            }
+            if constexpr (NTS) {  // C is written once and not read here: keep it out of the caches
+                __builtin_nontemporal_store(vr, cre + m_c.rowbase + r);
+                __builtin_nontemporal_store(vi, cim + m_c.rowbase + r);
             } else {
-                cre[m_c.rowbase + r] = acc_r[j];
-                cim[m_c.rowbase + r] = acc_i[j];
+                cre[m_c.rowbase + r] = vr;
+                cim[m_c.rowbase + r] = vi;
             }
         }
     }
@@ -140,46 +235,94 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     if (A.cols != B.rows) throw ArgError("diaQMatrix product_cuda: dimension mismatch.");
     if (A.precision != B.precision) throw ArgError("product: operands have different precisions");
     const DryRun d = dryrun(A.rows, B.cols, A.offs, B.offs);
-    auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s);
-    if (d.res.empty()) return C;
-    // work items, row-chunk major (chunks aligned to global multiples of PROD_CHUNK)
-    std::vector<WorkItem> work;
-    int64_t max_q = 0;
-    for (int c = 0; c < C->n(); ++c) max_q = std::max(max_q, (C->meta[c].rhi + PROD_CHUNK - 1) / PROD_CHUNK);
-    std::vector<std::vector<WorkItem>> by_q(static_cast<size_t>(max_q));
-    for (int c = 0; c < C->n(); ++c) {
-        const DiagMeta &m = C->meta[c];
-        if (m.rhi <= m.rlo) continue;
-        for (int64_t q = m.rlo / PROD_CHUNK; q * PROD_CHUNK < m.rhi; ++q) {
-            const int64_t r0 = std::max(m.rlo, q * PROD_CHUNK);
-            const int64_t r1 = std::min(m.rhi, (q + 1) * PROD_CHUNK);
-            by_q[q].push_back(WorkItem{r0, c, static_cast<int32_t>(r1 - r0)});
-        }
+    // every stored element of C is written by the kernel; only the gaps between diagonals are zeroed
+    auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s, false);
+    const int nc = C->n();
+    if (nc == 0) return C;
+    {
+        const unsigned g = blocks_for(nc + 1, 256, 1 << 20);
+        if (A.precision == DQ_F64)
+            hipLaunchKernelGGL(zero_gaps_kernel<double>, dim3(g), dim3(256), 0, s, nc, C->d_meta.as<DiagMeta>(),
+                               C->total, C->re.as<double>(), C->im.as<double>());
+        else
+            hipLaunchKernelGGL(zero_gaps_kernel<float>, dim3(g), dim3(256), 0, s, nc, C->d_meta.as<DiagMeta>(),
+                               C->total, C->re.as<float>(), C->im.as<float>());
+        DQ_LAUNCH_CHECK();
     }
-    for (auto &v : by_q) work.insert(work.end(), v.begin(), v.end());
-    if (work.empty()) return C;
-    const int64_t n_work = static_cast<int64_t>(work.size());
-    const int64_t per_xcd = (n_work + 7) / 8;
+    // per row chunk, the contiguous range of result diagonals that meet it (rlo and rhi do not increase with c).
+    // Chunk order: the chunks q with q mod 8 == 0 first, then those with q mod 8 == 1, ... (interleave: the XCDs work
+    // on neighbouring chunks at the same time, so a far pair's B rows, which are another chunk's near rows, are read
+    // from the Infinity Cache), or plain ascending order.
+    int64_t max_rhi = 0;
+    for (int c = 0; c < nc; ++c) max_rhi = std::max(max_rhi, C->meta[c].rhi);
+    const int64_t n_chunks = (max_rhi + PROD_CHUNK - 1) / PROD_CHUNK;
+    std::vector<int64_t> order;
+    order.reserve(static_cast<size_t>(n_chunks));
+    std::vector<int64_t> xcd_first(9, 0);
+    for (int x = 0; x < 8; ++x) {
+        if (g_prod_interleave)
+            for (int64_t q = x; q < n_chunks; q += 8) order.push_back(q);
+        else
+            for (int64_t q = n_chunks * x / 8; q < n_chunks * (x + 1) / 8; ++q) order.push_back(q);
+    }
+    std::vector<int64_t> first(static_cast<size_t>(n_chunks) + 1, 0);
+    std::vector<int32_t> c_lo(static_cast<size_t>(std::max<int64_t>(1, n_chunks)), 0), q_of(c_lo.size(), 0);
+    for (int64_t i = 0; i < n_chunks; ++i) {
+        const int64_t q = order[i];
+        const int64_t Q0 = q * PROD_CHUNK, Q1 = Q0 + PROD_CHUNK;
+        int lo = 0, hi = nc;  // first c with rlo < Q1
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (C->meta[mid].rlo >= Q1) lo = mid + 1; else hi = mid;
+        }
+        const int c1 = lo;
+        lo = c1; hi = nc;     // first c (from c1) with rhi <= Q0
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (C->meta[mid].rhi > Q0) lo = mid + 1; else hi = mid;
+        }
+        const int c2 = lo;
+        c_lo[i] = c1;
+        q_of[i] = static_cast<int32_t>(q);
+        first[i + 1] = first[i] + std::max(0, c2 - c1);
+    }
+    const int64_t n_work = first[n_chunks];
+    if (n_work == 0) return C;
+    // the item list (in chunk order) is split evenly over the XCDs: with many chunks each XCD works through whole
+    // chunks in the order above; with fewer chunks than XCDs a chunk's items spread over several
+    for (int x = 0; x <= 8; ++x) xcd_first[x] = n_work * x / 8;
+    int64_t per_xcd = 0;
+    for (int x = 0; x < 8; ++x) per_xcd = std::max(per_xcd, xcd_first[x + 1] - xcd_first[x]);
     const int64_t grid = per_xcd * 8;
-    DevBuf dw(sizeof(WorkItem) * work.size()), dps(sizeof(int64_t) * d.pair_start.size()),
+    if (grid > 0x7fffffffLL) throw ArgError("product: too many work items");
+    DevBuf dfirst(sizeof(int64_t) * first.size()), dclo(sizeof(int32_t) * c_lo.size()),
+        dqof(sizeof(int32_t) * q_of.size()), dxcd(sizeof(int64_t) * xcd_first.size()),
+        ditems(sizeof(int2) * static_cast<size_t>(n_work)), dps(sizeof(int64_t) * d.pair_start.size()),
         dsa(sizeof(int32_t) * std::max<size_t>(1, d.sa.size())), dsb(sizeof(int32_t) * std::max<size_t>(1, d.sb.size()));
-    DQ_HIP(hipMemcpyAsync(dw.p, work.data(), dw.bytes, hipMemcpyHostToDevice, s));
+    DQ_HIP(hipMemcpyAsync(dfirst.p, first.data(), dfirst.bytes, hipMemcpyHostToDevice, s));
+    DQ_HIP(hipMemcpyAsync(dclo.p, c_lo.data(), dclo.bytes, hipMemcpyHostToDevice, s));
+    DQ_HIP(hipMemcpyAsync(dqof.p, q_of.data(), dqof.bytes, hipMemcpyHostToDevice, s));
+    DQ_HIP(hipMemcpyAsync(dxcd.p, xcd_first.data(), dxcd.bytes, hipMemcpyHostToDevice, s));
     DQ_HIP(hipMemcpyAsync(dps.p, d.pair_start.data(), dps.bytes, hipMemcpyHostToDevice, s));
     DQ_HIP(hipMemcpyAsync(dsa.p, d.sa.data(), sizeof(int32_t) * d.sa.size(), hipMemcpyHostToDevice, s));
     DQ_HIP(hipMemcpyAsync(dsb.p, d.sb.data(), sizeof(int32_t) * d.sb.size(), hipMemcpyHostToDevice, s));
-    if (grid > 0x7fffffffLL) throw ArgError("product: too many work items");
-    if (A.precision == DQ_F64)
-        hipLaunchKernelGGL(product_kernel<double>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
-                           dw.as<WorkItem>(), n_work, per_xcd, dps.as<int64_t>(), dsa.as<int32_t>(),
-                           dsb.as<int32_t>(), A.d_meta.as<DiagMeta>(), B.d_meta.as<DiagMeta>(),
-                           C->d_meta.as<DiagMeta>(), A.re.as<double>(), A.im.as<double>(), B.re.as<double>(),
-                           B.im.as<double>(), C->re.as<double>(), C->im.as<double>(), divisor);
-    else
-        hipLaunchKernelGGL(product_kernel<float>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
-                           dw.as<WorkItem>(), n_work, per_xcd, dps.as<int64_t>(), dsa.as<int32_t>(),
-                           dsb.as<int32_t>(), A.d_meta.as<DiagMeta>(), B.d_meta.as<DiagMeta>(),
-                           C->d_meta.as<DiagMeta>(), A.re.as<float>(), A.im.as<float>(), B.re.as<float>(),
-                           B.im.as<float>(), C->re.as<float>(), C->im.as<float>(), divisor);
+    hipLaunchKernelGGL(product_items_kernel, dim3(static_cast<unsigned>(std::min<int64_t>(n_chunks, 65536))),
+                       dim3(256), 0, s, n_chunks, dfirst.as<int64_t>(), dclo.as<int32_t>(), dqof.as<int32_t>(),
+                       ditems.as<int2>());
+    DQ_LAUNCH_CHECK();
+#define DQ_PROD_LAUNCH(T, NTS)                                                                                     \
+    hipLaunchKernelGGL((product_kernel<T, NTS>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,              \
+                       ditems.as<int2>(), dxcd.as<int64_t>(), dps.as<int64_t>(), dsa.as<int32_t>(),               \
+                       dsb.as<int32_t>(), A.d_meta.as<DiagMeta>(), B.d_meta.as<DiagMeta>(), C->d_meta.as<DiagMeta>(), \
+                       A.re.as<T>(), A.im.as<T>(), B.re.as<T>(), B.im.as<T>(), C->re.as<T>(), C->im.as<T>(), divisor)
+    if (A.precision == DQ_F64) {
+        if (g_prod_nt_store) DQ_PROD_LAUNCH(double, true);
+        else DQ_PROD_LAUNCH(double, false);
+    } else {
+        if (g_prod_nt_store) DQ_PROD_LAUNCH(float, true);
+        else DQ_PROD_LAUNCH(float, false);
+    }
+#undef DQ_PROD_LAUNCH
     DQ_LAUNCH_CHECK();
     DQ_HIP(hipStreamSynchronize(s));  // before the work list is freed
     return C;
@@ -236,6 +379,23 @@ extern "C" int dq_product_dryrun(uint32_t rows, uint32_t cols, int32_t n_a, cons
         }
     });
 }
+
+namespace dq {
+// Called by dq_set_tuning for the "product_*" keys.
+bool product_set_tuning(const std::string &key, int value) {
+    if (key == "product_interleave") {
+        if (value < 0 || value > 1) throw ArgError("product_interleave must be 0 or 1");
+        g_prod_interleave = value;
+        return true;
+    }
+    if (key == "product_nt_store") {
+        if (value < 0 || value > 1) throw ArgError("product_nt_store must be 0 or 1");
+        g_prod_nt_store = value;
+        return true;
+    }
+    return false;
+}
+}  // namespace dq
 
 extern "C" int dq_product_device(const dq_devmat *A, const dq_devmat *B, dq_devmat **out) {
     return guard([&] {

@@ -118,7 +118,9 @@ int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontempora
  * 16), "spmv_bands" (0 = automatic, or the number of column bands), "spmv_nontemporal" (-1 = automatic, 0, 1:
  * stream the matrix values with non-temporal loads), "spmv_pipelined" (-1 = automatic, 0 = per-row predicated
  * loads, 1 = batched loads) and "spmv_lds" (-1 = automatic, 0, 1: stage the x window of each row tile in LDS for the
- * diagonals within half a tile). They override a matrix's tuned schedule. No reference counterpart. */
+ * diagonals within half a tile) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
+ * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
+ * product's values) set the SpGEMM kernel's schedule. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------

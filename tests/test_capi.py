@@ -55,6 +55,9 @@ def dryrun_capi(rows, cols, oa, ob):
     (6, 5, [-5, -2, 0, 3, 8], [-7, 0, 4]),       # A is 6 x 9, B is 9 x 5
     (1024, 1024, [3, -1, 0, 5, -1000], [1000, -3, 0]),  # unsorted slots
     (4, 4, [], [0, 1]),
+    (1 << 24, 1 << 24, [5, -(1 << 23), 0, 77, -3], [1 << 22, -77, 0, 3]),  # offset range too wide for bucket counts
+    (1 << 25, 3, [0, 2, -(1 << 24)], [1, 0, -1, 2]),                      # tall, wide range, out-of-range sums
+    (300, 300, list(range(-299, 300, 7)), list(range(-150, 151, 3))),     # many pairs per result diagonal
 ])
 def test_dryrun_bit_exact_vs_oracle(rows, cols, oa, ob):
     res, pairs = dryrun_capi(rows, cols, oa, ob)

@@ -309,6 +309,8 @@ def test_spgemm_reference_known_answer_pairs(d):
     ((600, 900, 700), [-599, -20, 0, 33, 899], [-899, -1, 0, 40, 699]),
     ((4096, 4096, 4096), list(range(-4, 5)), list(range(-4, 5))),
     ((64, 64, 64), [0, 63, -63], [-63, 63, 1]),
+    ((5000, 5000, 5000), [-4100, -2049, 0, 2047, 3000], [-3000, -1, 2, 2049]),  # diagonals start mid-chunk
+    ((600, 600, 600), list(range(-599, 600, 3)), list(range(-598, 599, 4))),    # > 256 pairs per result diagonal
 ])
 def test_product_device_structure_and_values(d, shapes, oa, ob):
     rng = np.random.default_rng(4)
