@@ -67,10 +67,10 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     # one-time schedule selection for this matrix (outside the timed region, like a library's plan step)
     best = ctypes.c_float()
     capi.check(L.dq_spmv_tune(out, 3, sid, ctypes.byref(best)))
-    cfg = [ctypes.c_int() for _ in range(4)]
+    cfg = [ctypes.c_int() for _ in range(5)]
     capi.check(L.dq_spmv_tuned_config(out, *[ctypes.byref(c) for c in cfg]))
     tuned = dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value, lds_window=cfg[3].value,
-                 tune_ms=best.value)
+                 xcd_tiles=cfg[4].value, tune_ms=best.value)
     log(f"[bench] spmv tuned schedule: {tuned}")
 
     def step():
@@ -249,7 +249,7 @@ def structured_spmv_section(args, torch, d, capi, Hm):
         sid = stream.cuda_stream
         best = ctypes.c_float()
         capi.check(L.dq_spmv_tune(A, 3, sid, ctypes.byref(best)))
-        cfg = [ctypes.c_int() for _ in range(4)]
+        cfg = [ctypes.c_int() for _ in range(5)]
         capi.check(L.dq_spmv_tuned_config(A, *[ctypes.byref(c) for c in cfg]))
         for _ in range(args.warmup):
             capi.check(L.dq_spmv_device(A, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
@@ -268,7 +268,7 @@ def structured_spmv_section(args, torch, d, capi, Hm):
                              "values (torch seed 2), complex128", N=N, n_diags=len(offs), stored_elements=S,
                     compulsory_bytes=nbytes, ms=ms, gbs=gbs, frac=gbs / HBM_PEAK_GBS,
                     schedule=dict(rows_per_lane=cfg[0].value, nontemporal=cfg[1].value, pipelined=cfg[2].value,
-                                  lds_window=cfg[3].value))
+                                  lds_window=cfg[3].value, xcd_tiles=cfg[4].value))
     finally:
         L.dq_devmat_destroy(A)
         torch.cuda.synchronize()
@@ -334,7 +334,7 @@ def expm_section(args, torch, d, capi, Hm):
                 seconds=t, n_diagonals=U.num_diags(), max_rel_err_vs_scipy_expm=err)
 
 
-def batched_section(args, torch, d, capi, Hm):
+def batched_section(args, torch, d, capi, Hm, quick=False):
     """256 TFIM Hamiltonians (n ~ U{10..18}, J ~ U[0.5, 1.5], h ~ U[0.2, 2]) x 8 complex64 unit states, each job
     evolved to t ~ log-uniform[0.01, 5]: the plain FP64 path (one fused evolution per state, (m, s) by the plain norm
     rule) against the batched FP32 plan with learned (m, s) (probe, fallback to the plain rule), both against an FP64
@@ -369,10 +369,11 @@ def batched_section(args, torch, d, capi, Hm):
         torch.cuda.synchronize()
         return out, time.perf_counter() - t0
 
-    ref, t_ref = fused([ev.plain_rule(nm, j["t"], 1e-12) for nm, j in zip(norms, jobs)], "FP64 reference")
-    plain_out, t_plain = fused(plain, "plain FP64")
-    err_plain = max(float(torch.max(torch.linalg.vector_norm(a - b, dim=1))) for a, b in zip(plain_out, ref))
-    log(f"[bench] plain FP64 path: {t_plain:.2f}s for {n_ev} evolutions, max state error {err_plain:.2e}")
+    if not quick:
+        ref, t_ref = fused([ev.plain_rule(nm, j["t"], 1e-12) for nm, j in zip(norms, jobs)], "FP64 reference")
+        plain_out, t_plain = fused(plain, "plain FP64")
+        err_plain = max(float(torch.max(torch.linalg.vector_norm(a - b, dim=1))) for a, b in zip(plain_out, ref))
+        log(f"[bench] plain FP64 path: {t_plain:.2f}s for {n_ev} evolutions, max state error {err_plain:.2e}")
 
     sel = S.Selector.load()
     B = ev.BatchedEvolution(Ds, 8, "fp32")
@@ -403,11 +404,18 @@ def batched_section(args, torch, d, capi, Hm):
     prop, final, est, gpu_ms = learned_run()
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
+    if quick:
+        log(f"[bench] batched FP32 + learned (m,s): {t_batched:.3f}s ({gpu_ms:.1f} ms graph)")
+        return dict(seconds=t_batched, graph_ms=gpu_ms)
     out_b = [torch.as_tensor(B.get_states(q, 1 << j["n"]), device="cuda") for q, j in enumerate(jobs)]
     err_b = max(float(torch.max(torch.linalg.vector_norm(a - b, dim=1))) for a, b in zip(out_b, ref))
     fallbacks = int(sum(1 for p, f in zip(prop, final) if p != f))
     spmv_plain = int(sum(m * 2 ** s for m, s in plain)) * 8
     spmv_learned = int(sum(m * 2 ** s for m, s in final)) * 8
+    # compulsory bytes of the graph's Horner levels (8 states): a level with k > 0 reads the FP32 values (8 B per
+    # stored element) and per state-row the gathered input, psi's FP32 copy, and writes the output (3 x 8 B); the
+    # last level of a sub-step also reads and writes psi in FP64 and writes its FP32 copy (+32 B)
+    graph_bytes = sum(2 ** s * (m * 8 * D.stored + 8 * D.shape[0] * (24 * m + 32)) for (m, s), D in zip(final, Ds))
     # the same plan with the plain schedule, to separate the batching/precision gain from the selector's
     load()
     B.set_schedule([j["t"] for j in jobs], [m for m, s in plain], [s for m, s in plain])
@@ -416,6 +424,7 @@ def batched_section(args, torch, d, capi, Hm):
     t1 = time.perf_counter()
     gpu_ms_plain_sched = B.run(use_graph=True)
     t_b_plain = time.perf_counter() - t1
+    log(f"[bench] batched graph: {graph_bytes / 1e9:.2f} GB compulsory, {graph_bytes / (gpu_ms * 1e-3) / 1e9:.0f} GB/s")
     log(f"[bench] batched FP32 + learned (m,s): {t_batched:.3f}s ({gpu_ms:.1f} ms graph), max state error "
         f"{err_b:.2e}, fallbacks {fallbacks}; same plan with the plain schedule {t_b_plain:.3f}s")
     return dict(
@@ -426,7 +435,8 @@ def batched_section(args, torch, d, capi, Hm):
                         spmv_per_state_total=spmv_plain),
         batched_fp32_learned=dict(seconds=t_batched, graph_ms=gpu_ms, evolutions_per_s=n_ev / t_batched,
                                   max_state_error=err_b, fallbacks=fallbacks, spmv_total=spmv_learned,
-                                  tuned_rows_per_lane=tuned),
+                                  tuned_rows_per_lane_xcd=tuned, graph_compulsory_bytes=graph_bytes,
+                                  graph_gbs=graph_bytes / (gpu_ms * 1e-3) / 1e9),
         batched_fp32_plain_schedule=dict(seconds=t_b_plain, graph_ms=gpu_ms_plain_sched,
                                          evolutions_per_s=n_ev / t_b_plain),
         speedup_vs_plain_fp64=t_plain / t_batched,
@@ -453,7 +463,7 @@ def main():
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
-                    choices=["spmv", "spmv_structured", "spgemm", "evolution"],
+                    choices=["spmv", "spmv_structured", "spgemm", "evolution", "batched"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -479,6 +489,9 @@ def main():
         elif args.profile == "spgemm":
             sg = spgemm_section(args, torch, d, capi, Hm, precisions=("fp64",))
             log(f"[bench] profile: spgemm {sg['fp64']['seconds'] * 1e3:.2f} ms")
+        elif args.profile == "batched":
+            bt = batched_section(args, torch, d, capi, Hm, quick=True)
+            log(f"[bench] profile: batched graph {bt['graph_ms']:.1f} ms")
         else:
             ev = evolution_section(args, torch, d, capi, Hm, steps=10, fp32=False, m100=False)
             log(f"[bench] profile: evolution {ev['plain_fp64']['seconds']:.3f} s")

@@ -34,6 +34,29 @@ struct Acc<float> {  // Kahan-compensated FP32 sum
     __device__ inline float get() const { return s; }
 };
 
+// Complex product (a_r + i a_i)(x_r + i x_i) with one fixed rounding sequence -- re = fma(a_r, x_r, -(a_i x_i)),
+// im = fma(a_r, x_i, a_i x_r) -- so every kernel instantiation and schedule (tile size, load order) rounds each term
+// identically; left to itself the compiler may contract a*b - c*d either way round.
+template <typename T>
+__device__ inline void cmul(T a_r, T a_i, T x_r, T x_i, T &re, T &im) {
+    re = __builtin_fma(a_r, x_r, -(a_i * x_i));
+    im = __builtin_fma(a_r, x_i, a_i * x_r);
+}
+template <>
+__device__ inline void cmul<float>(float a_r, float a_i, float x_r, float x_i, float &re, float &im) {
+    re = __builtin_fmaf(a_r, x_r, -(a_i * x_i));
+    im = __builtin_fmaf(a_r, x_i, a_i * x_r);
+}
+
+// Accumulate the complex product into a row's (real, imaginary) sums.
+template <typename T>
+__device__ inline void cmac(Acc<T> &sr, Acc<T> &si, T a_r, T a_i, T x_r, T x_i) {
+    T re, im;
+    cmul(a_r, a_i, x_r, x_i, re, im);
+    sr.add(re);
+    si.add(im);
+}
+
 // Loads of the matrix values: plain, or non-temporal (the values are streamed once, so they need not displace x from
 // L2 and the Infinity Cache).
 template <typename T, bool NT>
@@ -41,6 +64,18 @@ __device__ inline T ld_val(const T *p) {
     if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
+
+// Row tile of workgroup b. xcd = 0: tile b. xcd = 1: the workgroups of one XCD (b mod 8: the hardware hands
+// consecutive workgroups to the 8 XCDs in turn) take one contiguous eighth of the tiles, [x*per, (x+1)*per), so an
+// XCD's L2 serves the x rows its neighbouring diagonals share; the grid is then 8*per workgroups and the surplus ones
+// (tile >= ntiles) have no work.
+__device__ inline int64_t tile_of_block(int64_t b, int64_t ntiles, int xcd) {
+    if (!xcd) return b;
+    const int64_t per = (ntiles + 7) >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+inline int64_t grid_for_tiles(int64_t ntiles, int xcd) { return xcd ? 8 * ((ntiles + 7) >> 3) : ntiles; }
 
 // Copy x[tile - HALO, tile + TILE + HALO) into LDS (sr, si), zero outside [0, cols), then synchronise the workgroup.
 template <typename T, int RPT, int HALO>
@@ -62,7 +97,9 @@ __device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__res
 }
 
 // PIPE = true: each diagonal's 4*RPT loads are issued together (full tiles unconditionally, boundary tiles with
-// clamped addresses), the schedule that keeps the most bytes in flight. PIPE = false: per-row predicated loads, which
+// clamped addresses), the schedule that keeps the most bytes in flight. Every schedule adds exactly the products of
+// the rows on the diagonal, in slot order, so a row's sum has the same bits whatever the schedule (a compensated
+// FP32 sum is not left unchanged by adding a zero). PIPE = false: per-row predicated loads, which
 // the compiler issues row after row -- fewer loads in flight per wave but half the registers, so twice the waves;
 // with scattered offsets (no x reuse between diagonals) that variant can fill L2 faster.
 //
@@ -94,25 +131,24 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
                     vx_r[j] = sr[threadIdx.x + 256 * j + base];
                     vx_i[j] = si[threadIdx.x + 256 * j + base];
                 }
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
             } else {  // boundary tile: clamped rows stay inside the tile, so their LDS index stays in the window
 #pragma unroll
                 for (int j = 0; j < RPT; ++j) {
                     const int64_t r = tile + threadIdx.x + 256 * j;
-                    const bool in = r >= lo && r < hi;
                     const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
-                    const T a_r = ld_val<T, NT>(pr + rc), a_i = ld_val<T, NT>(pi + rc);
+                    va_r[j] = ld_val<T, NT>(pr + rc);
+                    va_i[j] = ld_val<T, NT>(pi + rc);
                     const int e = static_cast<int>(rc - tile) + base;
-                    const T x_r = sr[e], x_i = si[e];
-                    va_r[j] = in ? a_r : T(0);
-                    va_i[j] = in ? a_i : T(0);
-                    vx_r[j] = in ? x_r : T(0);
-                    vx_i[j] = in ? x_i : T(0);
+                    vx_r[j] = sr[e];
+                    vx_i[j] = si[e];
                 }
-            }
 #pragma unroll
-            for (int j = 0; j < RPT; ++j) {
-                ar[j].add(va_r[j] * vx_r[j] - va_i[j] * vx_i[j]);
-                ai[j].add(va_r[j] * vx_i[j] + va_i[j] * vx_r[j]);
+                for (int j = 0; j < RPT; ++j) {  // rows off the diagonal add nothing (not even a zero)
+                    const int64_t r = tile + threadIdx.x + 256 * j;
+                    if (r >= lo && r < hi) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+                }
             }
             continue;
         }
@@ -124,8 +160,7 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
                 const int64_t r = tile + threadIdx.x + 256 * j;
                 if (r >= lo && r < hi) {
                     const T a_r = ld_val<T, NT>(pr + r), a_i = ld_val<T, NT>(pi + r), x_r = qr[r], x_i = qi[r];
-                    ar[j].add(a_r * x_r - a_i * x_i);
-                    ai[j].add(a_r * x_i + a_i * x_r);
+                    cmac(ar[j], ai[j], a_r, a_i, x_r, x_i);
                 }
             }
             continue;
@@ -141,7 +176,9 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
                 vx_r[j] = qr[r];
                 vx_i[j] = qi[r];
             }
-        } else {  // boundary tile: clamp to a valid row, then zero both operands of the rows off the diagonal
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+        } else {  // boundary tile: clamp the loads to a valid row; rows off the diagonal add nothing (not even a zero)
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
                 const int64_t r = tile + threadIdx.x + 256 * j;
@@ -154,13 +191,8 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
                 const int64_t r = tile + threadIdx.x + 256 * j;
-                if (r < lo || r >= hi) { va_r[j] = T(0); va_i[j] = T(0); vx_r[j] = T(0); vx_i[j] = T(0); }
+                if (r >= lo && r < hi) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
             }
-        }
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            ar[j].add(va_r[j] * vx_r[j] - va_i[j] * vx_i[j]);
-            ai[j].add(va_r[j] * vx_i[j] + va_i[j] * vx_r[j]);
         }
     }
 }

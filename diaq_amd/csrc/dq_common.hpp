@@ -137,6 +137,8 @@ struct dq_devmat {
     // matrix loads (-1 = default), load schedule (-1 = default, 0 = per-row predicated, 1 = batched), LDS x window
     // (-1 = default, 0 = off, 1 = on).
     int tuned_rpt = 0, tuned_nt = -1, tuned_pipe = -1, tuned_lds = -1;
+    // Row-tile order chosen by dq_spmv_tune (-1 = default round-robin, 0 = round-robin, 1 = XCD-contiguous).
+    int tuned_xcd = -1;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {
@@ -174,6 +176,8 @@ std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
 
 // Product tuning keys ("product_interleave", "product_nt_store"); false if the key is not one of them.
 bool product_set_tuning(const std::string &key, int value);
+// Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd"); false if the key is not one of them.
+bool evo_set_tuning(const std::string &key, int value);
 
 // C = (A B) / divisor with the structural dry-run (product.hip).
 std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);
@@ -186,5 +190,8 @@ bool lds_window_pays(const dq_devmat &A, int rpt);
 // Whether row-tile kernels over A at rows-per-lane rpt stage the x window in LDS: the process-wide knob if set, else
 // the tuned choice, else lds_window_pays.
 bool lds_window_for(const dq_devmat &A, int rpt);
+// Row-tile order of row-tile kernels over A (0 = round-robin over the XCDs, 1 = XCD-contiguous): the process-wide knob
+// if set, else the tuned choice, else 0.
+int xcd_tiles_for(const dq_devmat &A);
 
 }  // namespace dq

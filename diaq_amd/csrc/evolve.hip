@@ -49,16 +49,19 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
                                                      int64_t g, double *__restrict__ p64r, double *__restrict__ p64i,
                                                      float *__restrict__ p32r, float *__restrict__ p32i,
                                                      const float *__restrict__ bin_r, const float *__restrict__ bin_i,
-                                                     float *__restrict__ bout_r, float *__restrict__ bout_i) {
+                                                     float *__restrict__ bout_r, float *__restrict__ bout_i,
+                                                     int64_t ntiles, int xcd) {
     const int j = blockIdx.y;
     const JobParam P = params[j];
     if (g >= P.steps) return;
+    const int64_t t = tile_of_block(blockIdx.x, ntiles, xcd);
+    if (t >= ntiles) return;  // uniform: a surplus workgroup of the XCD-contiguous grid
     const int m = P.m;
     const int k = m - 1 - static_cast<int>(g % m);  // Horner level
     constexpr int RPP = 256 / NS;                   // rows per pass of the workgroup
     const int st = threadIdx.x % NS;
     const int rl = threadIdx.x / NS;
-    const int64_t tile = static_cast<int64_t>(blockIdx.x) * RPP * RPT;
+    const int64_t tile = t * RPP * RPT;
     const int64_t vb = static_cast<int64_t>(j) * total;
     const int64_t sb = static_cast<int64_t>(j) * N * NS;
     // gathers always come from the previous level's output: b_{k+1}, or at the top level the FP32 state written by
@@ -84,11 +87,9 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
             vx_i[q] = static_cast<TV>(xi[xo]);
         }
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
+        for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
             const int64_t r = tile + rl + RPP * q;
-            if (r < lo || r >= hi_) { va_r[q] = TV(0); va_i[q] = TV(0); vx_r[q] = TV(0); vx_i[q] = TV(0); }
-            hr[q].add(va_r[q] * vx_r[q] - va_i[q] * vx_i[q]);
-            hi[q].add(va_r[q] * vx_i[q] + va_i[q] * vx_r[q]);
+            if (r >= lo && r < hi_) cmac(hr[q], hi[q], va_r[q], va_i[q], vx_r[q], vx_i[q]);
         }
     }
 #pragma unroll
@@ -149,9 +150,7 @@ __global__ __launch_bounds__(256) void probe_kernel(int64_t N, int n, const Diag
             TV a_r = ar[rc], a_i = ai[rc];
             const int64_t xo = (rc + md.off) * NS + st;
             TV x_r = static_cast<TV>(xr[xo]), x_i = static_cast<TV>(xi[xo]);
-            if (r < lo || r >= hi_) { a_r = TV(0); a_i = TV(0); x_r = TV(0); x_i = TV(0); }
-            hr[q].add(a_r * x_r - a_i * x_i);
-            hi[q].add(a_r * x_i + a_i * x_r);
+            if (r >= lo && r < hi_) cmac(hr[q], hi[q], a_r, a_i, x_r, x_i);
         }
     }
     const double c = P.tau / static_cast<double>(kstep);
@@ -264,6 +263,7 @@ struct EvoClass {
     std::vector<JobParam> params;
     int64_t G = 0;            // launches: max steps
     int rpt = 4;              // tuned rows per lane
+    int xcd = 0;              // tuned row-tile order: 0 round-robin over the XCDs, 1 XCD-contiguous
     hipStream_t stream = nullptr;
     int jobs_n() const { return static_cast<int>(jobs.size()); }
 };
@@ -289,12 +289,34 @@ struct dq_evo {
 
 namespace dq {
 
+// Process-wide overrides of every class's tuned Horner schedule (dq_set_tuning "evo_rows_per_lane", "evo_xcd").
+static int g_evo_rpt = 0;   // 0 = the class's, else 1, 2, 4 or 8
+static int g_evo_xcd = -1;  // -1 = the class's, else 0 or 1
+
+bool evo_set_tuning(const std::string &key, int value) {
+    if (key == "evo_rows_per_lane") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+            throw ArgError("evo_rows_per_lane must be 0, 1, 2, 4 or 8");
+        g_evo_rpt = value;
+        return true;
+    }
+    if (key == "evo_xcd") {
+        if (value < -1 || value > 1) throw ArgError("evo_xcd must be -1, 0 or 1");
+        g_evo_xcd = value;
+        return true;
+    }
+    return false;
+}
+
 template <typename TV, int NS>
 static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     const int64_t N = c.N;
     constexpr int RPP = 256 / NS;
-    const int64_t rows_per_block = static_cast<int64_t>(RPP) * c.rpt;
-    const dim3 grid(static_cast<unsigned>((N + rows_per_block - 1) / rows_per_block), static_cast<unsigned>(c.jobs_n()));
+    const int rpt = g_evo_rpt > 0 ? g_evo_rpt : c.rpt;
+    const int xcd = g_evo_xcd >= 0 ? g_evo_xcd : c.xcd;
+    const int64_t rows_per_block = static_cast<int64_t>(RPP) * rpt;
+    const int64_t ntiles = (N + rows_per_block - 1) / rows_per_block;
+    const dim3 grid(static_cast<unsigned>(grid_for_tiles(ntiles, xcd)), static_cast<unsigned>(c.jobs_n()));
     float *bout_r = (g & 1) ? c.b1r.as<float>() : c.b0r.as<float>();
     float *bout_i = (g & 1) ? c.b1i.as<float>() : c.b0i.as<float>();
     const float *bin_r = (g & 1) ? c.b0r.as<float>() : c.b1r.as<float>();
@@ -303,8 +325,8 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     hipLaunchKernelGGL((horner_kernel<TV, NS, R>), grid, dim3(256), 0, s, N, (int)c.offs.size(),                  \
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
                        g, c.p64r.as<double>(), c.p64i.as<double>(), c.p32r.as<float>(), c.p32i.as<float>(), bin_r,   \
-                       bin_i, bout_r, bout_i)
-    switch (c.rpt) {
+                       bin_i, bout_r, bout_i, ntiles, xcd)
+    switch (rpt) {
         case 1: DQ_HORNER(1); break;
         case 2: DQ_HORNER(2); break;
         case 4: DQ_HORNER(4); break;
@@ -534,8 +556,9 @@ extern "C" int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m,
     });
 }
 
-// Rows per lane of each class, chosen by timing a few Horner levels of each candidate (the schedule must be set).
-extern "C" int dq_evo_tune(dq_evo *E, int *chosen) {
+// Rows per lane and row-tile order of each class, chosen by timing a few Horner levels of each candidate (the
+// schedule must be set).
+extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd) {
     return guard([&] {
         if (!E) throw ArgError("plan is NULL");
         hipEvent_t e0, e1;
@@ -544,7 +567,7 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen) {
         for (size_t ci = 0; ci < E->classes.size(); ++ci) {
             EvoClass &c = E->classes[ci];
             float best = 1e30f;
-            int brpt = c.rpt;
+            int brpt = c.rpt, bxcd = c.xcd;
             // scratch copy of the states: tuning must not advance them
             const size_t sv = static_cast<size_t>(c.N) * E->n_states * c.jobs_n();
             DevBuf sr(sv * 8), si(sv * 8);
@@ -553,8 +576,10 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen) {
             DevBuf fr(sv * 4), fi(sv * 4);
             DQ_HIP(hipMemcpy(fr.p, c.p32r.p, sv * 4, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(fi.p, c.p32i.p, sv * 4, hipMemcpyDeviceToDevice));
-            for (int rpt : {1, 2, 4, 8}) {
+            for (int cand = 0; cand < 8; ++cand) {
+                const int rpt = 1 << (cand & 3), xcd = cand >> 2;
                 c.rpt = rpt;
+                c.xcd = xcd;
                 dispatch(E->value_precision, E->n_states, [&](auto tv, auto ns) {
                     using TV = decltype(tv);
                     constexpr int NS = decltype(ns)::value;
@@ -566,10 +591,12 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen) {
                 DQ_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; brpt = rpt; }
+                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; }
             }
             c.rpt = brpt;
-            if (chosen) chosen[ci] = brpt;
+            c.xcd = bxcd;
+            if (chosen_rpt) chosen_rpt[ci] = brpt;
+            if (chosen_xcd) chosen_xcd[ci] = bxcd;
             DQ_HIP(hipMemcpy(c.p64r.p, sr.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p64i.p, si.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p32r.p, fr.p, sv * 4, hipMemcpyDeviceToDevice));

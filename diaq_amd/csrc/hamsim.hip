@@ -25,12 +25,14 @@ __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, c
                                                           const T *__restrict__ tr, const T *__restrict__ ti,
                                                           T *__restrict__ nr, T *__restrict__ ni,
                                                           T *__restrict__ outr, T *__restrict__ outi, double scale,
-                                                          int first) {
+                                                          int first, int64_t ntiles, int xcd) {
     constexpr int TILE = 256 * RPT;
     constexpr int HALO = LDS ? TILE / 2 : 0;
     constexpr int W = LDS ? TILE + 2 * HALO : 1;
     __shared__ T sr[W], si[W];
-    const int64_t tile = static_cast<int64_t>(blockIdx.x) * TILE;
+    const int64_t t = tile_of_block(blockIdx.x, ntiles, xcd);
+    if (t >= ntiles) return;  // uniform: a surplus workgroup of the XCD-contiguous grid
+    const int64_t tile = t * TILE;
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
     if constexpr (LDS) stage_x_window<T, RPT, HALO>(tile, rows, tr, ti, sr, si);
     Acc<T> hr[RPT], hi[RPT];
@@ -75,7 +77,9 @@ static void taylor_step(const dq_devmat &H, const T *pr, const T *pi, T *outr, T
         return;
     }
     constexpr int RPT = 8;
-    const unsigned grid = blocks_for(N, 256 * RPT, 1LL << 31);
+    const int64_t ntiles = (N + 256 * RPT - 1) / (256 * RPT);
+    const int xcd = xcd_tiles_for(H);
+    const unsigned grid = static_cast<unsigned>(grid_for_tiles(ntiles, xcd));
     const bool lds = lds_window_for(H, RPT);
     const T *inr = pr, *ini = pi;
     T *bufr[2] = {t0r, t1r}, *bufi[2] = {t0i, t1i};
@@ -85,12 +89,12 @@ static void taylor_step(const dq_devmat &H, const T *pr, const T *pi, T *outr, T
             hipLaunchKernelGGL((taylor_term_kernel<T, RPT, true>), dim3(grid), dim3(256), 0, s, N, H.n(),
                                H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
                                static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
-                               dt / static_cast<double>(k), k == 1 ? 1 : 0);
+                               dt / static_cast<double>(k), k == 1 ? 1 : 0, ntiles, xcd);
         else
             hipLaunchKernelGGL((taylor_term_kernel<T, RPT, false>), dim3(grid), dim3(256), 0, s, N, H.n(),
                                H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
                                static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
-                               dt / static_cast<double>(k), k == 1 ? 1 : 0);
+                               dt / static_cast<double>(k), k == 1 ? 1 : 0, ntiles, xcd);
         DQ_LAUNCH_CHECK();
         inr = nr;
         ini = ni;

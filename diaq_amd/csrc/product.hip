@@ -22,7 +22,7 @@
 #include <algorithm>
 #include <numeric>
 
-#include "dq_common.hpp"
+#include "dia_rows.hpp"
 
 namespace dq {
 
@@ -205,8 +205,10 @@ __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ i
             }
 #pragma unroll
             for (int j = 0; j < PROD_RPT; ++j) {
-                acc_r[j] += va_r[j] * vb_r[j] - va_i[j] * vb_i[j];
-                acc_i[j] += va_r[j] * vb_i[j] + va_i[j] * vb_r[j];
+                T re, im;
+                cmul(va_r[j], va_i[j], vb_r[j], vb_i[j], re, im);
+                acc_r[j] += re;
+                acc_i[j] += im;
             }
         }
         __syncthreads();  // the next batch overwrites the staged pairs

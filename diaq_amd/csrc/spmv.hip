@@ -34,6 +34,7 @@ static int g_spmv_bands = 0;  // column bands; 0 = automatic
 static int g_spmv_nt = -1;    // non-temporal matrix loads: -1 = automatic, 0 = off, 1 = on
 static int g_spmv_pipe = -1;  // load schedule: -1 = automatic, 0 = per-row predicated, 1 = batched
 static int g_spmv_lds = -1;   // LDS x window: -1 = automatic, 0 = off, 1 = on
+static int g_spmv_xcd = -1;   // tile order: -1 = automatic, 0 = round-robin over the XCDs, 1 = XCD-contiguous
 
 template <typename T, int RPT, bool NT, bool PIPE, bool LDS>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t cols, int n,
@@ -41,12 +42,14 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t co
                                                         const T *__restrict__ aim, const T *__restrict__ xr,
                                                         const T *__restrict__ xi, T *__restrict__ yr,
                                                         T *__restrict__ yi, int64_t col_lo, int64_t col_hi,
-                                                        int accumulate) {
+                                                        int accumulate, int64_t ntiles, int xcd) {
     constexpr int TILE = 256 * RPT;
     constexpr int HALO = LDS ? TILE / 2 : 0;
     constexpr int W = LDS ? TILE + 2 * HALO : 1;
     __shared__ T sr[W], si[W];
-    const int64_t tile = static_cast<int64_t>(blockIdx.x) * TILE;
+    const int64_t t = tile_of_block(blockIdx.x, ntiles, xcd);
+    if (t >= ntiles) return;  // uniform: a surplus workgroup of the XCD-contiguous grid
+    const int64_t tile = t * TILE;
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
     if constexpr (LDS) stage_x_window<T, RPT, HALO>(tile, cols, xr, xi, sr, si);
     Acc<T> ar[RPT], ai[RPT];
@@ -67,13 +70,16 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t co
 
 template <typename T>
 static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *yi, int rpt, bool nt, bool pipe,
-                        bool lds, int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
+                        bool lds, int xcd, int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
     const int64_t rows = A.rows, cols = A.cols;
     const DiagMeta *meta = A.d_meta.as<DiagMeta>();
     const T *are = static_cast<const T *>(A.re.p), *aim = static_cast<const T *>(A.im.p);
+    const int64_t ntiles = (rows + 256 * rpt - 1) / (256 * rpt);
+    const int64_t grid = grid_for_tiles(ntiles, xcd);
+    if (grid > 0x7fffffffLL) throw ArgError("spmv: too many rows");
 #define DQ_SPMV_LAUNCH(R, NT, PIPE, LDS)                                                                         \
-    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE, LDS>), dim3(blocks_for(rows, 256 * R, 1LL << 31)),     \
-                       dim3(256), 0, s, rows, cols, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate)
+    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE, LDS>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, \
+                       rows, cols, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate, ntiles, xcd)
 #define DQ_SPMV_SCHED(R, LDS)                                         \
     if (nt && pipe) DQ_SPMV_LAUNCH(R, true, true, LDS);               \
     else if (nt) DQ_SPMV_LAUNCH(R, true, false, LDS);                 \
@@ -138,6 +144,12 @@ bool lds_window_pays(const dq_devmat &A, int rpt) {
     return near >= 2;
 }
 
+int xcd_tiles_for(const dq_devmat &A) {
+    if (g_spmv_xcd >= 0) return g_spmv_xcd;
+    if (A.tuned_xcd >= 0) return A.tuned_xcd;
+    return 0;
+}
+
 bool lds_window_for(const dq_devmat &A, int rpt) {
     if (g_spmv_lds >= 0) return g_spmv_lds == 1;
     if (A.tuned_lds >= 0) return A.tuned_lds == 1;
@@ -162,15 +174,16 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     if (n_bands > static_cast<int>(A.cols)) n_bands = std::max<int>(1, A.cols);
     const int rpt = auto_rpt(A);
     const bool nt = auto_nt(A), pipe = auto_pipe(A), lds = lds_window_for(A, rpt);
+    const int xcd = xcd_tiles_for(A);
     const int64_t cols = A.cols;
     for (int b = 0; b < n_bands; ++b) {
         const int64_t c0 = cols * b / n_bands, c1 = cols * (b + 1) / n_bands;
         if (A.precision == DQ_F64)
             launch_rows<double>(A, (const double *)xr, (const double *)xi, (double *)yr, (double *)yi, rpt, nt, pipe,
-                                lds, c0, c1, b > 0, s);
+                                lds, xcd, c0, c1, b > 0, s);
         else
             launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, nt, pipe, lds,
-                               c0, c1, b > 0, s);
+                               xcd, c0, c1, b > 0, s);
     }
 }
 
@@ -199,10 +212,13 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "spmv_lds") {
             if (value < -1 || value > 1) throw ArgError("spmv_lds must be -1, 0 or 1");
             g_spmv_lds = value;
+        } else if (k == "spmv_xcd") {
+            if (value < -1 || value > 1) throw ArgError("spmv_xcd must be -1, 0 or 1");
+            g_spmv_xcd = value;
         } else if (k == "spmv_bands") {
             if (value < 0 || value > 4096) throw ArgError("spmv_bands must be in [0, 4096]");
             g_spmv_bands = value;
-        } else if (!product_set_tuning(k, value)) {
+        } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value)) {
             throw ArgError("unknown tuning key: " + k);
         }
     });
@@ -225,47 +241,59 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
         DQ_HIP(hipEventCreate(&e0));
         DQ_HIP(hipEventCreate(&e1));
         float best = 1e30f;
-        int brpt = 8, bnt = 1, bpipe = 1, blds = 0;
-        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe, saved_lds = g_spmv_lds;
-        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1; g_spmv_lds = -1;
+        int brpt = 8, bnt = 1, bpipe = 1, blds = 0, bxcd = 0;
+        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe, saved_lds = g_spmv_lds,
+                  saved_xcd = g_spmv_xcd;
+        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1; g_spmv_lds = -1; g_spmv_xcd = -1;
+        auto time_it = [&]() {
+            spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);  // warm
+            DQ_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);
+            DQ_HIP(hipEventRecord(e1, s));
+            DQ_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms / repeats;
+        };
         try {
+            A->tuned_xcd = 0;
             for (int lds = 0; lds <= 1; ++lds)
             for (int pipe = 0; pipe <= 1; ++pipe)
                 for (int rpt : {2, 4, 8, 16})
                     for (int nt = 0; nt <= 1; ++nt) {
                         if (lds && !lds_window_pays(*A, rpt) && !lds_window_pays(*A, 16)) continue;
                         A->tuned_rpt = rpt; A->tuned_nt = nt; A->tuned_pipe = pipe; A->tuned_lds = lds;
-                        spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);  // warm
-                        DQ_HIP(hipEventRecord(e0, s));
-                        for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);
-                        DQ_HIP(hipEventRecord(e1, s));
-                        DQ_HIP(hipEventSynchronize(e1));
-                        float ms = 0.f;
-                        DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        ms /= repeats;
+                        const float ms = time_it();
                         if (ms < best) { best = ms; brpt = rpt; bnt = nt; bpipe = pipe; blds = lds; }
                     }
+            // second stage: the winner with the XCD-contiguous tile order
+            A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds; A->tuned_xcd = 1;
+            const float ms = time_it();
+            if (ms < best) { best = ms; bxcd = 1; }
         } catch (...) {
             g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
+            g_spmv_xcd = saved_xcd;
             (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
             throw;
         }
         g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
+        g_spmv_xcd = saved_xcd;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds;
+        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds; A->tuned_xcd = bxcd;
         if (best_ms) *best_ms = best;
     });
 }
 
 extern "C" int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined,
-                                    int *lds_window) {
+                                    int *lds_window, int *xcd_tiles) {
     return guard([&] {
         if (!A) throw ArgError("matrix is NULL");
         if (rows_per_lane) *rows_per_lane = auto_rpt(*A);
         if (nontemporal) *nontemporal = auto_nt(*A) ? 1 : 0;
         if (pipelined) *pipelined = auto_pipe(*A) ? 1 : 0;
         if (lds_window) *lds_window = lds_window_for(*A, auto_rpt(*A)) ? 1 : 0;
+        if (xcd_tiles) *xcd_tiles = xcd_tiles_for(*A);
     });
 }
 

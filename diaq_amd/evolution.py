@@ -89,9 +89,11 @@ class BatchedEvolution:
                                                   capi.ptr(s, ctypes.c_int32)))
 
     def tune(self):
-        out = (ctypes.c_int * self.n_classes)()
-        capi.check(capi.lib().dq_evo_tune(self._h, out))
-        return list(out)
+        """Per class: (rows per lane, XCD-contiguous tile order) chosen by timing."""
+        rpt = (ctypes.c_int * self.n_classes)()
+        xcd = (ctypes.c_int * self.n_classes)()
+        capi.check(capi.lib().dq_evo_tune(self._h, rpt, xcd))
+        return list(zip(list(rpt), list(xcd)))
 
     def probe(self) -> np.ndarray:
         est = np.empty(self.n_jobs)

@@ -112,15 +112,18 @@ int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 
 /* The SpMV schedule A currently uses (after tuning or the defaults); any output pointer may be NULL. No reference
  * counterpart. */
-int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined, int *lds_window);
+int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined, int *lds_window,
+                         int *xcd_tiles);
 
 /* Process-wide tuning knobs for the autotuner and benchmarks: "spmv_rows_per_lane" (0 = automatic, or 1, 2, 4, 8,
  * 16), "spmv_bands" (0 = automatic, or the number of column bands), "spmv_nontemporal" (-1 = automatic, 0, 1:
  * stream the matrix values with non-temporal loads), "spmv_pipelined" (-1 = automatic, 0 = per-row predicated
- * loads, 1 = batched loads) and "spmv_lds" (-1 = automatic, 0, 1: stage the x window of each row tile in LDS for the
- * diagonals within half a tile) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
+ * loads, 1 = batched loads), "spmv_lds" (-1 = automatic, 0, 1: stage the x window of each row tile in LDS for the
+ * diagonals within half a tile) and "spmv_xcd" (-1 = automatic, 0 = row tiles round-robin over the XCDs, 1 = each
+ * XCD a contiguous eighth of the tiles) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
- * product's values) set the SpGEMM kernel's schedule. No reference counterpart. */
+ * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8) and "evo_xcd"
+ * (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------
@@ -227,9 +230,10 @@ int dq_evo_get_states(const dq_evo *E, int job, double *re, double *im);
 /* Per job: evolve to t[j] by 2^s[j] sub-steps of a degree-m[j] Taylor polynomial (1 <= m <= 64, 0 <= s <= 24). */
 int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int32_t *s);
 
-/* Rows per lane of every class by timing the candidates (after the schedule is set); chosen (may be NULL) receives
- * one value per class. */
-int dq_evo_tune(dq_evo *E, int *chosen);
+/* Rows per lane and row-tile order (0 = round-robin over the XCDs, 1 = each XCD a contiguous eighth of the rows) of
+ * every class by timing the candidates (after the schedule is set); chosen_rpt / chosen_xcd (may be NULL) receive one
+ * value per class. */
+int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd);
 
 /* A-posteriori truncation estimate per job from its current (initial) states: 2^s ||t_{m+1}|| / (1 - ratio) of the
  * first sub-step's Taylor terms, the largest over the job's states. Leaves the states unchanged. */

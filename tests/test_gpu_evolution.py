@@ -90,3 +90,38 @@ def test_probe_flags_too_small_degree(mods):
     B.set_schedule([j["t"] for j in jobs], [2] * len(jobs), [0] * len(jobs))
     est_bad = B.probe()
     assert np.all(est_bad > 1e-6), est_bad
+
+
+def test_every_horner_schedule_gives_the_same_bits(mods):
+    """Rows per lane and the row-tile order only change which workgroup computes a row, not the row's arithmetic: every
+    (rows per lane, XCD order) schedule, forced through the process-wide knobs, reproduces the default run bit for bit,
+    and so does the tuned plan."""
+    from diaq_amd import capi
+    d, ev = mods
+    jobs = small_batch(d, seed=3, n_states=4)
+    sched = [ev.plain_rule(d.estimate_1norm(j["D"]), j["t"], 1e-8) for j in jobs]
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=4, value_precision="fp32")
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in sched], [s for m, s in sched])
+    L = capi.lib()
+
+    def run():
+        for q, j in enumerate(jobs):
+            B.set_states(q, j["states"])
+        B.run(use_graph=False, renormalize=True)
+        return [B.get_states(q, 1 << j["n"]) for q, j in enumerate(jobs)]
+
+    base = run()
+    try:
+        for rpt in (1, 2, 4, 8):
+            for xcd in (0, 1):
+                capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+                capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
+                for a, b in zip(run(), base):
+                    np.testing.assert_array_equal(a, b, err_msg=f"rpt={rpt} xcd={xcd}")
+    finally:
+        capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
+        capi.check(L.dq_set_tuning(b"evo_xcd", -1))
+    chosen = B.tune()
+    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) for r, x in chosen)
+    for a, b in zip(run(), base):
+        np.testing.assert_array_equal(a, b)

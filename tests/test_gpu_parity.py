@@ -198,18 +198,20 @@ def test_spmv_every_schedule(d, shape, offs):
                 for nt in (0, 1):
                     for pipe in (0, 1):
                         for lds in (0, 1):
-                            for key, v in (("spmv_rows_per_lane", rpt), ("spmv_nontemporal", nt),
-                                           ("spmv_pipelined", pipe), ("spmv_lds", lds)):
-                                capi.check(L.dq_set_tuning(key.encode(), v))
-                            yr = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
-                            yi = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
-                            capi.check(L.dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(),
-                                                        yi.data_ptr(), 0))
-                            torch.cuda.synchronize()
-                            got = (yr.double() + 1j * yi.double()).cpu().numpy()
-                            assert rel(got, ref) < tol, (prec, rpt, nt, pipe, lds)
+                            for xcd in (0, 1):
+                                for key, v in (("spmv_rows_per_lane", rpt), ("spmv_nontemporal", nt),
+                                               ("spmv_pipelined", pipe), ("spmv_lds", lds), ("spmv_xcd", xcd)):
+                                    capi.check(L.dq_set_tuning(key.encode(), v))
+                                yr = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
+                                yi = torch.full((shape[0],), float("nan"), dtype=dt, device="cuda")
+                                capi.check(L.dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(),
+                                                            yi.data_ptr(), 0))
+                                torch.cuda.synchronize()
+                                got = (yr.double() + 1j * yi.double()).cpu().numpy()
+                                assert rel(got, ref) < tol, (prec, rpt, nt, pipe, lds, xcd)
     finally:
-        for key, v in (("spmv_rows_per_lane", 0), ("spmv_nontemporal", -1), ("spmv_pipelined", -1), ("spmv_lds", -1)):
+        for key, v in (("spmv_rows_per_lane", 0), ("spmv_nontemporal", -1), ("spmv_pipelined", -1), ("spmv_lds", -1),
+                       ("spmv_xcd", -1)):
             capi.check(L.dq_set_tuning(key.encode(), v))
 
 
@@ -229,10 +231,10 @@ def test_spmv_tuner_keeps_results(d):
         D = d.to_device(dia_of(d, N, N, offs, vals))
         best = ctypes.c_float()
         capi.check(L.dq_spmv_tune(D.handle, 2, 0, ctypes.byref(best)))
-        cfg = [ctypes.c_int(-5) for _ in range(4)]
+        cfg = [ctypes.c_int(-5) for _ in range(5)]
         capi.check(L.dq_spmv_tuned_config(D.handle, *[ctypes.byref(c) for c in cfg]))
         assert cfg[0].value in (2, 4, 8, 16) and cfg[1].value in (0, 1) and cfg[2].value in (0, 1)
-        assert cfg[3].value in ((0, 1) if lds_possible else (0,))
+        assert cfg[3].value in ((0, 1) if lds_possible else (0,)) and cfg[4].value in (0, 1)
         assert best.value > 0
         got = d.to_numpy_vector(d.spMV(D, d.from_numpy_vector(x)))
         assert rel(got, O.spmv(M, x)) < TOL
@@ -440,14 +442,16 @@ def test_hamsim_vs_oracle(d):
     psi = Hm.product_state(10, 2)
     H = dia_of(d, rows, cols, offs, vals)
     try:
-        for lds in (0, 1, -1):  # term kernel without / with the LDS x window, then the automatic choice
+        for lds, xcd in ((0, 0), (1, 0), (0, 1), (1, 1), (-1, -1)):  # forced term-kernel schedules, then automatic
             capi.check(capi.lib().dq_set_tuning(b"spmv_lds", lds))
+            capi.check(capi.lib().dq_set_tuning(b"spmv_xcd", xcd))
             for t, r, m in ((0.5, 5, 12), (1.0, 3, 30)):
                 got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), t, r, m))
                 ref = O.hamiltonian_simulation(M, psi, t, r, m)
-                assert rel(got, ref) < TOL, (lds, t, r, m)
+                assert rel(got, ref) < TOL, (lds, xcd, t, r, m)
     finally:
         capi.check(capi.lib().dq_set_tuning(b"spmv_lds", -1))
+        capi.check(capi.lib().dq_set_tuning(b"spmv_xcd", -1))
     exact = scipy.sparse.linalg.expm_multiply(-1j * 1.0 * O.to_dense(M), psi)
     got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 10, 20))
     assert np.linalg.norm(got - exact) < 1e-10
