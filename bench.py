@@ -311,6 +311,45 @@ def spgemm_section(args, torch, d, capi, Hm, precisions=("fp64", "fp32")):
     return out
 
 
+def spmm_section(args, torch, d, capi, Hm):
+    """spmm_dia_device on torch tensors: C = A B with A in row-aligned DIA form (9 diagonals, offsets -4..4) of
+    N = 2^22 rows, B and C N x 8, complex128 (interleaved) -- time per call and compulsory bytes
+    16 * (n_diags * N + 2 * N * n_right)."""
+    N, n_right = 1 << 22, 8
+    offs = torch.arange(-4, 5, dtype=torch.int64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    vals = torch.randn(len(offs) * N, dtype=torch.complex128, device="cuda", generator=g)
+    B = torch.randn(N, n_right, dtype=torch.complex128, device="cuda", generator=g)
+    C = torch.empty(N, n_right, dtype=torch.complex128, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        d.spmm_dia_device(offs.data_ptr(), vals.data_ptr(), len(offs), N, N, B.data_ptr(), n_right, C.data_ptr(),
+                          True, stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = 16 * (len(offs) * N + 2 * N * n_right)
+    # spot check against a torch reference on a slice of rows
+    rows = torch.arange(1000, 1064, device="cuda")
+    ref = torch.zeros(len(rows), n_right, dtype=torch.complex128, device="cuda")
+    for k, o in enumerate(range(-4, 5)):
+        ref += vals[k * N + rows].unsqueeze(1) * B[rows + o]
+    err = float((C[rows] - ref).abs().max() / ref.abs().max())
+    log(f"[bench] spmm_dia_device: {ms:.3f} ms  {nbytes / (ms * 1e-3) / 1e9:.1f} GB/s  spot-check rel err {err:.1e}")
+    return dict(workload="spmm_dia_device, N=2^22 rows, 9 diagonals (-4..4), B and C N x 8, complex128", ms=ms,
+                compulsory_bytes=nbytes, gbs=nbytes / (ms * 1e-3) / 1e9, spot_check_rel_err=err)
+
+
 def expm_section(args, torch, d, capi, Hm):
     """expm(-i H t) at t = 0.1 for the 12-qubit TFIM (J = 1, h = 0.7) by Taylor with scaling and squaring on the GPU,
     against scipy's dense expm."""
@@ -463,7 +502,7 @@ def main():
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
-                    choices=["spmv", "spmv_structured", "spgemm", "evolution", "batched"],
+                    choices=["spmv", "spmv_structured", "spgemm", "spmm_dia", "evolution", "batched"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -489,6 +528,9 @@ def main():
         elif args.profile == "spgemm":
             sg = spgemm_section(args, torch, d, capi, Hm, precisions=("fp64",))
             log(f"[bench] profile: spgemm {sg['fp64']['seconds'] * 1e3:.2f} ms")
+        elif args.profile == "spmm_dia":
+            sm = spmm_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: spmm_dia {sm['ms']:.3f} ms")
         elif args.profile == "batched":
             bt = batched_section(args, torch, d, capi, Hm, quick=True)
             log(f"[bench] profile: batched graph {bt['graph_ms']:.1f} ms")
@@ -512,7 +554,7 @@ def main():
     extra = {}
     if not args.no_evolution:
         for key, fn in (("spmv_structured", structured_spmv_section), ("spgemm", spgemm_section),
-                        ("expm", expm_section)):
+                        ("spmm_dia", spmm_section), ("expm", expm_section)):
             try:
                 extra[key] = fn(args, torch, d, capi, Hm)
             except Exception as e:
@@ -552,6 +594,7 @@ def main():
         "spmv_structured": extra.get("spmv_structured"),
         "evolution": evo,
         "spgemm": extra.get("spgemm"),
+        "spmm_dia": extra.get("spmm_dia"),
         "expm": extra.get("expm"),
         "batched_evolution": bat,
     }

@@ -10,7 +10,9 @@
 // B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run of B and
 // writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles rows,
 // which coalesces only for wide B). The diagonal values vals[k * n_rows + r] are shared by the n_right lanes of a row.
-// Offsets are staged in LDS (up to 2048 of them). Each element sums its diagonals in order, as the reference does.
+// Each element sums its diagonals in order, as the reference does; the loads of 8 diagonals are issued before their
+// products are added, so a lane keeps 16 loads in flight, and a diagonal's offset and base pointers are uniform
+// (scalar) values.
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -19,7 +21,7 @@
 
 namespace dq {
 
-constexpr int SPMM_LDS_OFFSETS = 2048;
+constexpr int SPMM_UNROLL = 8;
 
 struct cplx {
     double x, y;
@@ -29,6 +31,20 @@ __device__ inline double fma_val(double acc, double a, double b) { return fma(a,
 __device__ inline cplx fma_val(cplx acc, cplx a, cplx b) {
     return cplx{acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
 }
+// The diagonal values are read once and C is written once: non-temporal, so B's rows keep the caches.
+typedef double d2v __attribute__((ext_vector_type(2)));  // one 16-byte access for a complex value
+__device__ inline double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ inline cplx ld_stream(const cplx *p) {
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(p));
+    return cplx{v.x, v.y};
+}
+__device__ inline void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ inline void st_stream(cplx *p, cplx v) {
+    d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<d2v *>(p));
+}
 template <typename T> __device__ inline T zero_val();
 template <> __device__ inline double zero_val<double>() { return 0.0; }
 template <> __device__ inline cplx zero_val<cplx>() { return cplx{0.0, 0.0}; }
@@ -37,24 +53,37 @@ template <typename T, typename I>
 __global__ __launch_bounds__(256) void spmm_dia_kernel(I n_rows, I n_cols, I n_right, int n_diags,
                                                        const int64_t *__restrict__ offsets, const T *__restrict__ vals,
                                                        const T *__restrict__ B, T *__restrict__ C) {
-    __shared__ int64_t s_off[SPMM_LDS_OFFSETS];
-    const bool staged = n_diags <= SPMM_LDS_OFFSETS;
-    if (staged) {
-        for (int k = threadIdx.x; k < n_diags; k += blockDim.x) s_off[k] = offsets[k];
-        __syncthreads();
-    }
     const I total = n_rows * n_right;
     for (I e = static_cast<I>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
          e += static_cast<I>(gridDim.x) * blockDim.x) {
         const I r = e / n_right;
         T acc = zero_val<T>();
-        for (int k = 0; k < n_diags; ++k) {
-            const int64_t o = staged ? s_off[k] : offsets[k];
-            const int64_t c = static_cast<int64_t>(r) + o;
-            if (c >= 0 && c < static_cast<int64_t>(n_cols))
-                acc = fma_val(acc, vals[static_cast<int64_t>(k) * n_rows + r], B[static_cast<int64_t>(e) + o * n_right]);
+        // SPMM_UNROLL diagonals at a time: their loads are issued together, then added in diagonal order. The
+        // diagonal's offset and base pointers are uniform (scalar loads and arithmetic); a lane adds only its row.
+        for (int k0 = 0; k0 < n_diags; k0 += SPMM_UNROLL) {
+            T a[SPMM_UNROLL], b[SPMM_UNROLL];
+            bool ok[SPMM_UNROLL];
+            int64_t o[SPMM_UNROLL];
+#pragma unroll
+            for (int u = 0; u < SPMM_UNROLL; ++u)  // all offsets first: independent scalar loads, one wait
+                o[u] = offsets[k0 + u < n_diags ? k0 + u : n_diags - 1];
+#pragma unroll
+            for (int u = 0; u < SPMM_UNROLL; ++u) {
+                const int k = k0 + u;
+                const int64_t c = static_cast<int64_t>(r) + o[u];
+                ok[u] = k < n_diags && c >= 0 && c < static_cast<int64_t>(n_cols);
+                a[u] = zero_val<T>();
+                b[u] = zero_val<T>();
+                if (ok[u]) {
+                    a[u] = ld_stream(vals + static_cast<int64_t>(k) * n_rows + r);
+                    b[u] = B[static_cast<int64_t>(e) + o[u] * static_cast<int64_t>(n_right)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SPMM_UNROLL; ++u)
+                if (ok[u]) acc = fma_val(acc, a[u], b[u]);
         }
-        C[e] = acc;
+        st_stream(C + e, acc);
     }
 }
 
@@ -63,7 +92,7 @@ static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags,
                    const T *B, T *C, hipStream_t s) {
     const int64_t total = n_rows * n_right;
     if (total == 0) return;
-    const unsigned grid = blocks_for(total, 256, 1 << 16);
+    const unsigned grid = blocks_for(total, 256, 1 << 20);
     if (total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31))
         hipLaunchKernelGGL((spmm_dia_kernel<T, uint32_t>), dim3(grid), dim3(256), 0, s, (uint32_t)n_rows,
                            (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C);
