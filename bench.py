@@ -474,7 +474,7 @@ def batched_section(args, torch, d, capi, Hm, quick=False):
                         spmv_per_state_total=spmv_plain),
         batched_fp32_learned=dict(seconds=t_batched, graph_ms=gpu_ms, evolutions_per_s=n_ev / t_batched,
                                   max_state_error=err_b, fallbacks=fallbacks, spmv_total=spmv_learned,
-                                  tuned_rows_per_lane_xcd=tuned, graph_compulsory_bytes=graph_bytes,
+                                  tuned_rows_per_lane_xcd_lds=tuned, graph_compulsory_bytes=graph_bytes,
                                   graph_gbs=graph_bytes / (gpu_ms * 1e-3) / 1e9),
         batched_fp32_plain_schedule=dict(seconds=t_b_plain, graph_ms=gpu_ms_plain_sched,
                                          evolutions_per_s=n_ev / t_b_plain),

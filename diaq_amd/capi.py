@@ -72,7 +72,7 @@ SIGNATURES = {
     "dq_evo_set_states": (c_int, [c_void_p, c_int, P(c_double), P(c_double)]),
     "dq_evo_get_states": (c_int, [c_void_p, c_int, P(c_double), P(c_double)]),
     "dq_evo_set_schedule": (c_int, [c_void_p, P(c_double), P(c_int32), P(c_int32)]),
-    "dq_evo_tune": (c_int, [c_void_p, P(c_int), P(c_int)]),
+    "dq_evo_tune": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
     "dq_evo_run": (c_int, [c_void_p, c_int, c_int, c_uintptr, P(ctypes.c_float)]),
     "dq_evo_probe": (c_int, [c_void_p, P(c_double)]),
     "dq_mlp_create": (c_int, [c_int, c_int, c_int, c_int, c_uint64, P(c_void_p)]),

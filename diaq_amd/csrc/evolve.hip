@@ -42,7 +42,10 @@ struct AccT {
 };
 
 // One Horner level for every job of a class. blockIdx.y = job in class; tiles of 256/NS rows x RPT.
-template <typename TV, int NS, int RPT>
+// LDSW: the workgroup first stages the input rows [tile - T/2, tile + 3T/2) (T rows per tile, all NS states, zero
+// outside the matrix) in LDS, and the diagonals with |offset| <= T/2 gather from there: the +-2^k offsets of a qubit
+// Hamiltonian then share one copy of the 8-state vectors per tile.
+template <typename TV, int NS, int RPT, bool LDSW>
 __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
                                                      int64_t total, const TV *__restrict__ hre,
                                                      const TV *__restrict__ him, const JobParam *__restrict__ params,
@@ -68,23 +71,54 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
     // the previous sub-step's last level (or by the run's prologue); psi's own entries are read from p32 / p64
     const float *xr = bin_r + sb;
     const float *xi = bin_i + sb;
+    constexpr int T = RPP * RPT;
+    constexpr int H = LDSW ? T / 2 : 0;
+    constexpr int WE = LDSW ? (T + 2 * H) * NS : 1;  // staged elements per array
+    __shared__ float sxr[WE], sxi[WE];
+    if constexpr (LDSW) {
+        const int64_t ws = (tile - H) * NS;  // first staged element (row tile - H, state 0)
+        for (int e = threadIdx.x; e < WE; e += 256) {
+            const int64_t gi = ws + e;
+            float a = 0.f, b = 0.f;
+            if (gi >= 0 && gi < N * NS) {
+                a = xr[gi];
+                b = xi[gi];
+            }
+            sxr[e] = a;
+            sxi[e] = b;
+        }
+        __syncthreads();
+    }
     Acc<TV> hr[RPT], hi[RPT];
     for (int d = 0; d < n; ++d) {
         const DiagMeta md = meta[d];
         const int64_t lo = md.rlo, hi_ = md.rhi;
-        if (hi_ <= tile || lo >= tile + RPP * RPT) continue;  // uniform
+        if (hi_ <= tile || lo >= tile + T) continue;  // uniform
         const TV *ar = hre + vb + md.rowbase;
         const TV *ai = him + vb + md.rowbase;
         TV va_r[RPT], va_i[RPT], vx_r[RPT], vx_i[RPT];
+        if (LDSW && md.off >= -H && md.off <= H) {  // uniform: gather from the staged window
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int64_t r = tile + rl + RPP * q;
-            const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);
-            va_r[q] = ar[rc];
-            va_i[q] = ai[rc];
-            const int64_t xo = (rc + md.off) * NS + st;
-            vx_r[q] = static_cast<TV>(xr[xo]);
-            vx_i[q] = static_cast<TV>(xi[xo]);
+            for (int q = 0; q < RPT; ++q) {
+                const int64_t r = tile + rl + RPP * q;
+                const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);  // stays inside the tile
+                va_r[q] = ar[rc];
+                va_i[q] = ai[rc];
+                const int e = (static_cast<int>(rc - tile) + H + md.off) * NS + st;
+                vx_r[q] = static_cast<TV>(sxr[e]);
+                vx_i[q] = static_cast<TV>(sxi[e]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int64_t r = tile + rl + RPP * q;
+                const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);
+                va_r[q] = ar[rc];
+                va_i[q] = ai[rc];
+                const int64_t xo = (rc + md.off) * NS + st;
+                vx_r[q] = static_cast<TV>(xr[xo]);
+                vx_i[q] = static_cast<TV>(xi[xo]);
+            }
         }
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
@@ -264,6 +298,7 @@ struct EvoClass {
     int64_t G = 0;            // launches: max steps
     int rpt = 4;              // tuned rows per lane
     int xcd = 0;              // tuned row-tile order: 0 round-robin over the XCDs, 1 XCD-contiguous
+    int lds = 0;              // tuned: 1 = stage each tile's input window in LDS
     hipStream_t stream = nullptr;
     int jobs_n() const { return static_cast<int>(jobs.size()); }
 };
@@ -289,9 +324,11 @@ struct dq_evo {
 
 namespace dq {
 
-// Process-wide overrides of every class's tuned Horner schedule (dq_set_tuning "evo_rows_per_lane", "evo_xcd").
+// Process-wide overrides of every class's tuned Horner schedule (dq_set_tuning "evo_rows_per_lane", "evo_xcd",
+// "evo_lds").
 static int g_evo_rpt = 0;   // 0 = the class's, else 1, 2, 4 or 8
 static int g_evo_xcd = -1;  // -1 = the class's, else 0 or 1
+static int g_evo_lds = -1;  // -1 = the class's, else 0 or 1
 
 bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_rows_per_lane") {
@@ -305,6 +342,11 @@ bool evo_set_tuning(const std::string &key, int value) {
         g_evo_xcd = value;
         return true;
     }
+    if (key == "evo_lds") {
+        if (value < -1 || value > 1) throw ArgError("evo_lds must be -1, 0 or 1");
+        g_evo_lds = value;
+        return true;
+    }
     return false;
 }
 
@@ -314,6 +356,7 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     constexpr int RPP = 256 / NS;
     const int rpt = g_evo_rpt > 0 ? g_evo_rpt : c.rpt;
     const int xcd = g_evo_xcd >= 0 ? g_evo_xcd : c.xcd;
+    const bool lds = (g_evo_lds >= 0 ? g_evo_lds : c.lds) == 1;
     const int64_t rows_per_block = static_cast<int64_t>(RPP) * rpt;
     const int64_t ntiles = (N + rows_per_block - 1) / rows_per_block;
     const dim3 grid(static_cast<unsigned>(grid_for_tiles(ntiles, xcd)), static_cast<unsigned>(c.jobs_n()));
@@ -321,11 +364,14 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     float *bout_i = (g & 1) ? c.b1i.as<float>() : c.b0i.as<float>();
     const float *bin_r = (g & 1) ? c.b0r.as<float>() : c.b1r.as<float>();
     const float *bin_i = (g & 1) ? c.b0i.as<float>() : c.b1i.as<float>();
-#define DQ_HORNER(R)                                                                                             \
-    hipLaunchKernelGGL((horner_kernel<TV, NS, R>), grid, dim3(256), 0, s, N, (int)c.offs.size(),                  \
+#define DQ_HORNER_L(R, L)                                                                                        \
+    hipLaunchKernelGGL((horner_kernel<TV, NS, R, L>), grid, dim3(256), 0, s, N, (int)c.offs.size(),               \
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
                        g, c.p64r.as<double>(), c.p64i.as<double>(), c.p32r.as<float>(), c.p32i.as<float>(), bin_r,   \
                        bin_i, bout_r, bout_i, ntiles, xcd)
+#define DQ_HORNER(R)                         \
+    if (lds) { DQ_HORNER_L(R, true); }       \
+    else { DQ_HORNER_L(R, false); }
     switch (rpt) {
         case 1: DQ_HORNER(1); break;
         case 2: DQ_HORNER(2); break;
@@ -334,6 +380,7 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
         default: throw ArgError("evolve: rows per lane must be 1, 2, 4 or 8");
     }
 #undef DQ_HORNER
+#undef DQ_HORNER_L
     DQ_LAUNCH_CHECK();
 }
 
@@ -556,9 +603,9 @@ extern "C" int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m,
     });
 }
 
-// Rows per lane and row-tile order of each class, chosen by timing a few Horner levels of each candidate (the
-// schedule must be set).
-extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd) {
+// Rows per lane, row-tile order and LDS window of each class, chosen by timing a few Horner levels of each
+// candidate (the schedule must be set).
+extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *chosen_lds) {
     return guard([&] {
         if (!E) throw ArgError("plan is NULL");
         hipEvent_t e0, e1;
@@ -567,7 +614,7 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd) {
         for (size_t ci = 0; ci < E->classes.size(); ++ci) {
             EvoClass &c = E->classes[ci];
             float best = 1e30f;
-            int brpt = c.rpt, bxcd = c.xcd;
+            int brpt = c.rpt, bxcd = c.xcd, blds = c.lds;
             // scratch copy of the states: tuning must not advance them
             const size_t sv = static_cast<size_t>(c.N) * E->n_states * c.jobs_n();
             DevBuf sr(sv * 8), si(sv * 8);
@@ -576,10 +623,11 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd) {
             DevBuf fr(sv * 4), fi(sv * 4);
             DQ_HIP(hipMemcpy(fr.p, c.p32r.p, sv * 4, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(fi.p, c.p32i.p, sv * 4, hipMemcpyDeviceToDevice));
-            for (int cand = 0; cand < 8; ++cand) {
-                const int rpt = 1 << (cand & 3), xcd = cand >> 2;
+            for (int cand = 0; cand < 16; ++cand) {
+                const int rpt = 1 << (cand & 3), xcd = (cand >> 2) & 1, lds = cand >> 3;
                 c.rpt = rpt;
                 c.xcd = xcd;
+                c.lds = lds;
                 dispatch(E->value_precision, E->n_states, [&](auto tv, auto ns) {
                     using TV = decltype(tv);
                     constexpr int NS = decltype(ns)::value;
@@ -591,12 +639,14 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd) {
                 DQ_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; }
+                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; blds = lds; }
             }
             c.rpt = brpt;
             c.xcd = bxcd;
+            c.lds = blds;
             if (chosen_rpt) chosen_rpt[ci] = brpt;
             if (chosen_xcd) chosen_xcd[ci] = bxcd;
+            if (chosen_lds) chosen_lds[ci] = blds;
             DQ_HIP(hipMemcpy(c.p64r.p, sr.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p64i.p, si.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p32r.p, fr.p, sv * 4, hipMemcpyDeviceToDevice));

@@ -89,11 +89,12 @@ class BatchedEvolution:
                                                   capi.ptr(s, ctypes.c_int32)))
 
     def tune(self):
-        """Per class: (rows per lane, XCD-contiguous tile order) chosen by timing."""
+        """Per class: (rows per lane, XCD-contiguous tile order, LDS window) chosen by timing."""
         rpt = (ctypes.c_int * self.n_classes)()
         xcd = (ctypes.c_int * self.n_classes)()
-        capi.check(capi.lib().dq_evo_tune(self._h, rpt, xcd))
-        return list(zip(list(rpt), list(xcd)))
+        lds = (ctypes.c_int * self.n_classes)()
+        capi.check(capi.lib().dq_evo_tune(self._h, rpt, xcd, lds))
+        return list(zip(list(rpt), list(xcd), list(lds)))
 
     def probe(self) -> np.ndarray:
         est = np.empty(self.n_jobs)

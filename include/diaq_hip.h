@@ -122,8 +122,8 @@ int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontempora
  * diagonals within half a tile) and "spmv_xcd" (-1 = automatic, 0 = row tiles round-robin over the XCDs, 1 = each
  * XCD a contiguous eighth of the tiles) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
- * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8) and "evo_xcd"
- * (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule. No reference counterpart. */
+ * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
+ * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------
@@ -230,10 +230,10 @@ int dq_evo_get_states(const dq_evo *E, int job, double *re, double *im);
 /* Per job: evolve to t[j] by 2^s[j] sub-steps of a degree-m[j] Taylor polynomial (1 <= m <= 64, 0 <= s <= 24). */
 int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int32_t *s);
 
-/* Rows per lane and row-tile order (0 = round-robin over the XCDs, 1 = each XCD a contiguous eighth of the rows) of
- * every class by timing the candidates (after the schedule is set); chosen_rpt / chosen_xcd (may be NULL) receive one
- * value per class. */
-int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd);
+/* Rows per lane, row-tile order (0 = round-robin over the XCDs, 1 = each XCD a contiguous eighth of the rows) and
+ * LDS window (1 = each tile stages its input rows +-T/2 in LDS) of every class by timing the candidates (after the
+ * schedule is set); chosen_rpt / chosen_xcd / chosen_lds (may be NULL) receive one value per class. */
+int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *chosen_lds);
 
 /* A-posteriori truncation estimate per job from its current (initial) states: 2^s ||t_{m+1}|| / (1 - ratio) of the
  * first sub-step's Taylor terms, the largest over the job's states. Leaves the states unchanged. */

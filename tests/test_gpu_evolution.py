@@ -93,9 +93,9 @@ def test_probe_flags_too_small_degree(mods):
 
 
 def test_every_horner_schedule_gives_the_same_bits(mods):
-    """Rows per lane and the row-tile order only change which workgroup computes a row, not the row's arithmetic: every
-    (rows per lane, XCD order) schedule, forced through the process-wide knobs, reproduces the default run bit for bit,
-    and so does the tuned plan."""
+    """Rows per lane, the row-tile order and the LDS window only change which workgroup computes a row and where its
+    inputs come from, not the row's arithmetic: every (rows per lane, XCD order, LDS window) schedule, forced through
+    the process-wide knobs, reproduces the default run bit for bit, and so does the tuned plan."""
     from diaq_amd import capi
     d, ev = mods
     jobs = small_batch(d, seed=3, n_states=4)
@@ -114,14 +114,18 @@ def test_every_horner_schedule_gives_the_same_bits(mods):
     try:
         for rpt in (1, 2, 4, 8):
             for xcd in (0, 1):
-                capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
-                capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
-                for a, b in zip(run(), base):
-                    np.testing.assert_array_equal(a, b, err_msg=f"rpt={rpt} xcd={xcd}")
+                for lds in (0, 1):
+                    capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+                    capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
+                    capi.check(L.dq_set_tuning(b"evo_lds", lds))
+                    for a, b in zip(run(), base):
+                        np.testing.assert_array_equal(a, b, err_msg=f"rpt={rpt} xcd={xcd} lds={lds}")
     finally:
         capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
         capi.check(L.dq_set_tuning(b"evo_xcd", -1))
+        capi.check(L.dq_set_tuning(b"evo_lds", -1))
     chosen = B.tune()
-    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) for r, x in chosen)
+    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) and w in (0, 1)
+                                              for r, x, w in chosen)
     for a, b in zip(run(), base):
         np.testing.assert_array_equal(a, b)
