@@ -203,6 +203,13 @@ def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True)
     S = H.stored
     out["bytes_per_term"] = 16 * S + 64 * rows
     out["plain_fp64"]["gbs"] = out["bytes_per_term"] * m_plain * steps / out["plain_fp64"]["seconds"] / 1e9
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = evolution_cpu_baseline(torch, d, H, offs, vals, rows, psi0, dt, m_plain, steps, sid,
+                                                         out["bytes_per_term"])
+        except Exception as e:
+            out["cpu_baseline"] = dict(error=str(e))
+            log(f"[bench] evolution cpu baseline failed: {e}")
     if not fp32:
         return out
     # precision form: the same evolution with FP32 values and vectors (Kahan-compensated row sums)
@@ -221,6 +228,29 @@ def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True)
                        speedup_vs_plain_fp64=out["plain_fp64"]["seconds"] / t)
     log(f"[bench] evolution fp32: m={m_plain} {t:.3f} s/evolution, error vs FP64 {out['fp32']['l2_err_vs_plain_fp64']:.1e}")
     return out
+
+
+def evolution_cpu_baseline(torch, d, H, offs, vals, rows, psi0, dt, m, steps, sid, bytes_per_term):
+    """One Taylor step of the same evolution on this host's cores with the reference's CPU SpMV (restated by the C
+    oracle), scaled to the full number of steps; checked against one GPU step from the same state."""
+    from oracle import coracle
+    nt = cpu_threads()
+    Hp = coracle.packed(offs, vals)
+    t0 = time.perf_counter()
+    cpu = coracle.taylor_step_tasks(rows, Hp, psi0, dt, m, n_threads=nt)
+    t1 = time.perf_counter() - t0
+    pr = torch.as_tensor(psi0.real, device="cuda").contiguous()
+    pi = torch.as_tensor(psi0.imag, device="cuda").contiguous()
+    d.hamsim_device(H, pr.data_ptr(), pi.data_ptr(), dt, 1, m, sid)
+    torch.cuda.synchronize()
+    g = (pr + 1j * pi).cpu().numpy()
+    diff = float(np.max(np.abs(g - cpu)) / np.max(np.abs(cpu)))
+    per_ev = t1 * steps
+    log(f"[bench] evolution cpu baseline: {t1:.2f} s per step, {per_ev:.1f} s per evolution (x{steps}), {nt} threads, "
+        f"GPU-vs-CPU max rel diff {diff:.1e}")
+    return dict(value=bytes_per_term * m / t1 / 1e9, unit="GB/s", seconds_per_step=t1, seconds_per_evolution=per_ev,
+                cores=nt, kind="port", sample=f"1 of the {steps} steps (m={m} terms), scaled by {steps}",
+                gpu_vs_cpu_max_rel_diff=diff)
 
 
 def structured_spmv_section(args, torch, d, capi, Hm):
@@ -302,6 +332,12 @@ def spgemm_section(args, torch, d, capi, Hm, precisions=("fp64", "fp32")):
                          gbs=nbytes / t / 1e9)
         log(f"[bench] spgemm {prec}: {t * 1e3:.2f} ms  {nbytes / t / 1e9:.1f} GB/s  {len(C.diagonals)} offsets")
         prods[prec] = C
+    if "fp64" in prods and not args.no_cpu:
+        try:
+            out["cpu_baseline"] = spgemm_cpu_baseline(offs, vals, r, prods["fp64"], out["fp64"]["compulsory_bytes"])
+        except Exception as e:
+            out["cpu_baseline"] = dict(error=str(e))
+            log(f"[bench] spgemm cpu baseline failed: {e}")
     if "fp32" not in prods:
         return out
     out["offsets_identical"] = list(prods["fp64"].diagonals) == list(prods["fp32"].diagonals)
@@ -309,6 +345,35 @@ def spgemm_section(args, torch, d, capi, Hm, precisions=("fp64", "fp32")):
     log(f"[bench] spgemm fp32 vs fp64: rel Frobenius {out['fp32_rel_frobenius_err']:.2e}, offsets identical "
         f"{out['offsets_identical']}")
     return out
+
+
+def spgemm_cpu_baseline(offs, vals, n_rows, C_gpu, nbytes):
+    """The reference's CPU SpGEMM (restated by the C oracle: OpenMP over tiles of 4 result diagonals x 2048
+    positions, pairs in order) on the same TFIM-20 H*H, this host's cores; checked against the GPU result on three
+    result diagonals."""
+    from oracle import coracle
+    from oracle import oracle as O
+    nt = cpu_threads()
+    A = coracle.packed(offs, vals)
+    res, pairs = O.product_dryrun(offs, offs, n_rows, n_rows)
+    ps = np.zeros(len(res) + 1, np.int64)
+    ps[1:] = np.cumsum([len(p) for p in pairs])
+    sa = np.array([a for p in pairs for a, _ in p], np.int32)
+    sb = np.array([b for p in pairs for _, b in p], np.int32)
+    t0 = time.perf_counter()
+    out = coracle.product_tasks(n_rows, n_rows, n_rows, A, A, (res, ps, sa, sb), n_threads=nt)
+    t = time.perf_counter() - t0
+    worst = 0.0
+    for o in (0, 1, res[0]):
+        k = list(out[0]).index(o)
+        L = O.diagonal_length(int(o), n_rows, n_rows)
+        cpu = out[2][out[1][k]:out[1][k] + L] + 1j * out[3][out[1][k]:out[1][k] + L]
+        g = C_gpu.get_diagonal(int(o))
+        worst = max(worst, float(np.max(np.abs(np.asarray(g) - cpu)) / max(1e-300, np.max(np.abs(cpu)))))
+    log(f"[bench] spgemm cpu baseline: {t:.2f} s ({nbytes / t / 1e9:.1f} GB/s), {nt} threads, GPU-vs-CPU max rel diff "
+        f"{worst:.1e}")
+    return dict(value=nbytes / t / 1e9, unit="GB/s", seconds=t, cores=nt, kind="port",
+                sample="the full TFIM-20 H*H (same values), one run", gpu_vs_cpu_max_rel_diff=worst)
 
 
 def spmm_section(args, torch, d, capi, Hm):

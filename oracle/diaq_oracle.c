@@ -189,3 +189,102 @@ void oracle_product_diag(int64_t rows_a, int64_t cols_a, int64_t cols_b, int n_a
         c_im[t] = ci;
     }
 }
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * The reference's CPU SpGEMM restated (src/spGEMM.cpp:303-514 with run_avx512<double>, src/simd_avx512_kernels.cpp:
+ * 20-128): OpenMP over tiles of res_block result diagonals (CPU_RES_BLOCK = 4, src/spGEMM.cpp:19-24); inside a tile,
+ * element blocks of elem_block positions (CPU_ELEM_BLOCK = 2048); per result diagonal its pairs in order, each adding
+ * A_a * B_b over the positions both diagonals cover -- a contiguous, vectorised complex multiply-add. C (packed at
+ * st_c) must be zeroed by the caller, as the reference's result is. The pair list is the structural dry-run.
+ * In position space the pair maps result position t (row i = t + max(-dc, 0)) to A position t + ca and B position
+ * t + cb, ca = i0 + min(da, 0), cb = i0 + (db >= 0 ? da : dc), i0 = max(-dc, 0): the reference's case analysis of
+ * multiplyDiagonalsSegment (src/spGEMM.cpp:53-300) computes the same alignment.
+ * ------------------------------------------------------------------------------------------------------------- */
+__attribute__((target_clones("avx512f", "avx2", "default")))
+static void cmadd_kernel(int64_t lo, int64_t hi, int64_t ca, int64_t cb, const double *restrict ar,
+                         const double *restrict ai, const double *restrict br, const double *restrict bi,
+                         double *restrict cr, double *restrict ci) {
+    for (int64_t t = lo; t < hi; ++t) {
+        const double a_r = ar[t + ca], a_i = ai[t + ca], b_r = br[t + cb], b_i = bi[t + cb];
+        cr[t] += a_r * b_r - a_i * b_i;
+        ci[t] += a_r * b_i + a_i * b_r;
+    }
+}
+
+void oracle_product_tasks(int64_t rows_a, int64_t cols_a, int64_t cols_b, const int32_t *off_a, const int64_t *st_a,
+                          const double *a_re, const double *a_im, const int32_t *off_b, const int64_t *st_b,
+                          const double *b_re, const double *b_im, int n_res, const int32_t *res_off,
+                          const int64_t *pair_start, const int32_t *slot_a, const int32_t *slot_b,
+                          const int64_t *st_c, double *c_re, double *c_im, int res_block, int elem_block,
+                          int n_threads) {
+    if (res_block <= 0) res_block = 4;
+    if (elem_block <= 0) elem_block = 2048;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+    const int n_tiles = (n_res + res_block - 1) / res_block;
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+    for (int tile = 0; tile < n_tiles; ++tile) {
+        const int r0 = tile * res_block, r1 = r0 + res_block < n_res ? r0 + res_block : n_res;
+        int64_t max_len = 0;
+        for (int rs = r0; rs < r1; ++rs) {
+            const int64_t l = diag_len(res_off[rs], rows_a, cols_b);
+            if (l > max_len) max_len = l;
+        }
+        for (int64_t j0 = 0; j0 < max_len; j0 += elem_block) {
+            const int64_t j1 = j0 + elem_block < max_len ? j0 + elem_block : max_len;
+            for (int rs = r0; rs < r1; ++rs) {
+                const int dc = res_off[rs];
+                const int64_t len_c = diag_len(dc, rows_a, cols_b);
+                if (j0 >= len_c) continue;
+                const int64_t je = j1 < len_c ? j1 : len_c;
+                const int64_t i0 = dc < 0 ? -(int64_t)dc : 0;
+                for (int64_t p = pair_start[rs]; p < pair_start[rs + 1]; ++p) {
+                    const int sa = slot_a[p], sb = slot_b[p];
+                    const int da = off_a[sa], db = off_b[sb];
+                    const int64_t ca = i0 + (da < 0 ? da : 0);
+                    const int64_t cb = i0 + (db >= 0 ? da : dc);
+                    const int64_t len_a = diag_len(da, rows_a, cols_a), len_b = diag_len(db, cols_a, cols_b);
+                    int64_t lo = j0, hi = je;
+                    if (lo < -ca) lo = -ca;
+                    if (lo < -cb) lo = -cb;
+                    if (hi > len_a - ca) hi = len_a - ca;
+                    if (hi > len_b - cb) hi = len_b - cb;
+                    if (hi <= lo) continue;
+                    cmadd_kernel(lo, hi, ca, cb, a_re + st_a[sa], a_im + st_a[sa], b_re + st_b[sb], b_im + st_b[sb],
+                                 c_re + st_c[rs], c_im + st_c[rs]);
+                }
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * One fixed-degree Taylor step on the CPU with the reference's CPU SpMV (oracle_spmv_tasks above):
+ * out = sum_{k=0}^{m} (-i dt H)^k psi / k!, term_k = (-i dt/k) H term_{k-1} (the arithmetic of
+ * src/spHamSim_gpu_all_fused.cu:235-277, each SpMV the reference's CPU kernel). Scratch: 4*rows doubles.
+ * ------------------------------------------------------------------------------------------------------------- */
+int oracle_taylor_step_tasks(int64_t rows, int n_diags, const int32_t *offsets, const int64_t *starts,
+                             const double *re, const double *im, const double *pr, const double *pi, double dt, int m,
+                             double *outr, double *outi, double *scratch, int n_threads) {
+    double *tr = scratch, *ti = scratch + rows, *hr = scratch + 2 * rows, *hi = scratch + 3 * rows;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+    for (int64_t i = 0; i < rows; ++i) {
+        tr[i] = pr[i];
+        ti[i] = pi[i];
+        outr[i] = pr[i];
+        outi[i] = pi[i];
+    }
+    for (int k = 1; k <= m; ++k) {
+        if (oracle_spmv_tasks(rows, rows, n_diags, offsets, starts, re, im, tr, ti, hr, hi, 0, n_threads)) return -1;
+        const double s = dt / (double)k;
+#pragma omp parallel for schedule(static) num_threads(n_threads)
+        for (int64_t i = 0; i < rows; ++i) {
+            const double vr = hi[i] * s, vi = -hr[i] * s; /* (-i s)(a + ib) */
+            tr[i] = vr;
+            ti[i] = vi;
+            outr[i] += vr;
+            outi[i] += vi;
+        }
+    }
+    return 0;
+}

@@ -332,6 +332,48 @@ def test_c_oracle_spmv_matches_numpy():
         np.testing.assert_allclose(yr + 1j * yi, ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("shapes,oa,ob", [
+    ((300, 300, 300), [-299, -40, -1, 0, 1, 7, 299], [-299, -5, 0, 2, 150, 299]),
+    ((200, 350, 120), [-199, -3, 0, 4, 349], [-349, -20, 0, 6, 119]),   # rectangular: A 200x350, B 350x120
+    ((5000, 5000, 5000), [-2100, -1, 0, 1, 2100], [-4097, 0, 3, 2049]),  # several element blocks per diagonal
+])
+def test_c_oracle_product_tasks_matches_dense(shapes, oa, ob):
+    """The restated reference CPU SpGEMM (tiles of 4 result diagonals x 2048 positions) against dense products."""
+    from oracle import coracle
+    m, k, n = shapes
+    rng = np.random.default_rng(12)
+    A = rand_dia(m, k, oa, rng)
+    B = rand_dia(k, n, ob, rng)
+    res, pairs = O.product_dryrun(oa, ob, m, n)
+    ps = np.zeros(len(res) + 1, np.int64)
+    ps[1:] = np.cumsum([len(p) for p in pairs])
+    sa = np.array([a for p in pairs for a, _ in p], np.int32)
+    sb = np.array([b for p in pairs for _, b in p], np.int32)
+    out = coracle.product_tasks(m, k, n, coracle.packed(A.offsets, A.vals), coracle.packed(B.offsets, B.vals),
+                                (res, ps, sa, sb), n_threads=3)
+    C = O.Dia(m, n, list(out[0]), [out[2][s:s + O.diagonal_length(o, m, n)] + 1j * out[3][s:s + O.diagonal_length(o, m, n)]
+                                   for o, s in zip(out[0], out[1])])
+    if m * n <= 4_000_000:
+        ref = O.to_dense(A) @ O.to_dense(B)
+        np.testing.assert_allclose(O.to_dense(C), ref, rtol=1e-12, atol=1e-12)
+    else:  # large: against the oracle's pair-by-pair product
+        R = O.product(A, B)
+        assert R.offsets == C.offsets
+        for a, b in zip(R.vals, C.vals):
+            np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12)
+
+
+def test_c_oracle_taylor_step_matches_oracle():
+    """One fixed-degree Taylor step with the restated reference CPU SpMV against the oracle's evolution rule."""
+    from oracle import coracle
+    rows, cols, offs, vals = Hm.xxz(9, seed=4)
+    M = O.Dia(rows, cols, offs, vals)
+    psi = Hm.product_state(9, 3)
+    got = coracle.taylor_step_tasks(rows, coracle.packed(offs, vals), psi, 0.3, 14, n_threads=2)
+    ref = O.hamiltonian_simulation(M, psi, 0.3, 1, 14)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-13)
+
+
 def test_c_oracle_block_size_rule():
     from oracle import coracle
     b = coracle.block_size()
