@@ -176,7 +176,9 @@ std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
 
 // Product tuning keys ("product_interleave", "product_nt_store"); false if the key is not one of them.
 bool product_set_tuning(const std::string &key, int value);
-// Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd"); false if the key is not one of them.
+// spmm_dia tuning key ("spmm_tiled"); false if it is another key.
+bool spmm_set_tuning(const std::string &key, int value);
+// Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd", "evo_lds"); false if the key is not one of them.
 bool evo_set_tuning(const std::string &key, int value);
 
 // C = (A B) / divisor with the structural dry-run (product.hip).

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "dq_common.hpp"
 
@@ -87,13 +88,104 @@ __global__ __launch_bounds__(256) void spmm_dia_kernel(I n_rows, I n_cols, I n_r
     }
 }
 
+// Tiled form: workgroup b owns rows [r0, r0 + TR) with TR = SPMM_TILE / n_right (SPMM_EPT elements per lane). When
+// two or more diagonals have |offset| <= TR/2 it first stages B's rows [r0 - TR/2, r0 + 3 TR/2) (zero outside B) in
+// LDS and those diagonals read B from there; the others read it from memory. Each element still sums its diagonals
+// in order.
+template <typename T, typename I, int SPMM_EPT>
+__global__ __launch_bounds__(256) void spmm_dia_tiled_kernel(I n_rows, I n_cols, I n_right, int n_diags,
+                                                             const int64_t *__restrict__ offsets,
+                                                             const T *__restrict__ vals, const T *__restrict__ B,
+                                                             T *__restrict__ C, I rows_per_tile) {
+    __shared__ T sb[2 * 256 * SPMM_EPT];
+    const int64_t TR = rows_per_tile;
+    // the window pays only when two or more diagonals fall inside it (uniform: every lane reads the same offsets)
+    int near = 0;
+    for (int k = 0; k < n_diags; ++k) {
+        const int64_t o = offsets[k];
+        near += (o >= -(TR / 2) && o <= TR / 2) ? 1 : 0;
+    }
+    const int64_t H = near >= 2 ? TR / 2 : -1;  // H < 0: no window, every diagonal reads B from memory
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * TR;
+    if (H >= 0) {
+        const int64_t W = (TR + 2 * H) * static_cast<int64_t>(n_right);  // <= 2 * 256 * SPMM_EPT
+        const int64_t b0 = (r0 - H) * static_cast<int64_t>(n_right);
+        const int64_t nb = static_cast<int64_t>(n_cols) * n_right;
+        for (int64_t e = threadIdx.x; e < W; e += 256) {
+            const int64_t gi = b0 + e;
+            T v = zero_val<T>();
+            if (gi >= 0 && gi < nb) v = B[gi];
+            sb[e] = v;
+        }
+        __syncthreads();
+    }
+    const int64_t rows_here = min(TR, static_cast<int64_t>(n_rows) - r0);
+    const I elems = static_cast<I>(rows_here * n_right);
+#pragma unroll
+    for (int u = 0; u < SPMM_EPT; ++u) {
+        const I local = static_cast<I>(threadIdx.x + 256 * u);
+        if (local >= elems) break;
+        const I lr = local / n_right;
+        const I r = static_cast<I>(r0) + lr;
+        const int64_t e = static_cast<int64_t>(r0) * n_right + local;
+        T acc = zero_val<T>();
+        for (int k0 = 0; k0 < n_diags; k0 += SPMM_UNROLL) {
+            T a[SPMM_UNROLL], b[SPMM_UNROLL];
+            bool ok[SPMM_UNROLL];
+            int64_t o[SPMM_UNROLL];
+#pragma unroll
+            for (int w = 0; w < SPMM_UNROLL; ++w) o[w] = offsets[k0 + w < n_diags ? k0 + w : n_diags - 1];
+#pragma unroll
+            for (int w = 0; w < SPMM_UNROLL; ++w) {
+                const int k = k0 + w;
+                const int64_t c = static_cast<int64_t>(r) + o[w];
+                ok[w] = k < n_diags && c >= 0 && c < static_cast<int64_t>(n_cols);
+                a[w] = zero_val<T>();
+                b[w] = zero_val<T>();
+                if (ok[w]) {
+                    a[w] = ld_stream(vals + static_cast<int64_t>(k) * n_rows + r);
+                    if (o[w] >= -H && o[w] <= H)  // uniform
+                        b[w] = sb[local + (H + o[w]) * static_cast<int64_t>(n_right)];
+                    else
+                        b[w] = B[e + o[w] * static_cast<int64_t>(n_right)];
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < SPMM_UNROLL; ++w)
+                if (ok[w]) acc = fma_val(acc, a[w], b[w]);
+        }
+        st_stream(C + e, acc);
+    }
+}
+
 template <typename T>
 static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off, const T *vals,
-                   const T *B, T *C, hipStream_t s) {
+                   const T *B, T *C, int ept, hipStream_t s) {
     const int64_t total = n_rows * n_right;
     if (total == 0) return;
+    const bool small = total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31);
+    if (ept > 0 && n_right <= 256 * ept) {
+        const int64_t tr = 256 * ept / n_right;
+        const int64_t tiles = (n_rows + tr - 1) / tr;
+        if (tiles > 0x7fffffffLL) throw ArgError("spmm_dia_device: too many rows");
+#define DQ_SPMM_TILED(E)                                                                                          \
+    if (small)                                                                                                    \
+        hipLaunchKernelGGL((spmm_dia_tiled_kernel<T, uint32_t, E>), dim3(static_cast<unsigned>(tiles)), dim3(256),  \
+                           0, s, (uint32_t)n_rows, (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C,   \
+                           (uint32_t)tr);                                                                         \
+    else                                                                                                          \
+        hipLaunchKernelGGL((spmm_dia_tiled_kernel<T, int64_t, E>), dim3(static_cast<unsigned>(tiles)), dim3(256),   \
+                           0, s, n_rows, n_cols, n_right, n_diags, off, vals, B, C, tr);
+        if (ept == 1) { DQ_SPMM_TILED(1) }
+        else if (ept == 2) { DQ_SPMM_TILED(2) }
+        else { DQ_SPMM_TILED(4) }
+#undef DQ_SPMM_TILED
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) throw HipError(std::string("spmm_dia_device: ") + hipGetErrorString(err));
+        return;
+    }
     const unsigned grid = blocks_for(total, 256, 1 << 20);
-    if (total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31))
+    if (small)
         hipLaunchKernelGGL((spmm_dia_kernel<T, uint32_t>), dim3(grid), dim3(256), 0, s, (uint32_t)n_rows,
                            (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C);
     else
@@ -102,6 +194,18 @@ static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags,
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) throw HipError(std::string("spmm_dia_device: ") + hipGetErrorString(err));
 }
+
+// Process-wide knob (dq_set_tuning "spmm_tiled"): 0 = flat kernel (default), 1, 2 or 4 = the tiled kernel with that
+// many elements per lane.
+static int g_spmm_tiled = 0;
+
+bool spmm_set_tuning(const std::string &key, int value) {
+    if (key != "spmm_tiled") return false;
+    if (value != 0 && value != 1 && value != 2 && value != 4) throw ArgError("spmm_tiled must be 0, 1, 2 or 4");
+    g_spmm_tiled = value;
+    return true;
+}
+
 
 }  // namespace dq
 
@@ -115,12 +219,14 @@ extern "C" int dq_spmm_dia_device(uintptr_t offsets, uintptr_t vals, int n_diags
         if (!C || (n_diags > 0 && (!offsets || !vals)) || (n_cols > 0 && !B))
             throw ArgError("spmm_dia_device: NULL pointer");
         const auto *off = reinterpret_cast<const int64_t *>(offsets);
+        const int ept = g_spmm_tiled;  // 0: flat kernel; 1, 2, 4: tiled, elements per lane
         if (is_complex)
             launch<cplx>(n_rows, n_cols, n_right, n_diags, off, reinterpret_cast<const cplx *>(vals),
-                         reinterpret_cast<const cplx *>(B), reinterpret_cast<cplx *>(C), as_stream(stream));
+                         reinterpret_cast<const cplx *>(B), reinterpret_cast<cplx *>(C), ept, as_stream(stream));
         else
             launch<double>(n_rows, n_cols, n_right, n_diags, off, reinterpret_cast<const double *>(vals),
-                           reinterpret_cast<const double *>(B), reinterpret_cast<double *>(C), as_stream(stream));
+                           reinterpret_cast<const double *>(B), reinterpret_cast<double *>(C), ept,
+                           as_stream(stream));
     });
 }
 

@@ -218,7 +218,7 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "spmv_bands") {
             if (value < 0 || value > 4096) throw ArgError("spmv_bands must be in [0, 4096]");
             g_spmv_bands = value;
-        } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value)) {
+        } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value)) {
             throw ArgError("unknown tuning key: " + k);
         }
     });

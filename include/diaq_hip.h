@@ -123,7 +123,9 @@ int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontempora
  * XCD a contiguous eighth of the tiles) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
  * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
- * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule. No reference counterpart. */
+ * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
+ * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
+ * spmm_dia_device kernel. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------

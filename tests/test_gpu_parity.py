@@ -509,7 +509,46 @@ def test_spmm_dia_device(d, case, complex_):
     tB = torch.as_tensor(B, device="cuda").to(dt).contiguous()
     tC = torch.empty((M, R), dtype=dt, device="cuda")
     off = torch.tensor(offs, dtype=torch.int64, device="cuda")
-    d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(), complex_,
-                      torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    assert rel(tC.cpu().numpy(), ref) < 1e-13
+    from diaq_amd import capi
+    try:
+        for tiled in (0, 1, 2, 4):  # flat kernel, LDS-tiled kernel with 1, 2, 4 elements per lane
+            capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", tiled))
+            tC.fill_(float("nan"))
+            d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(),
+                              complex_, torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            assert rel(tC.cpu().numpy(), ref) < 1e-13, tiled
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
+
+
+@pytest.mark.parametrize("M,K,R,offs", [
+    (5000, 5000, 8, [-4100, -2, 0, 3, 600]),         # near and far diagonals, ragged last tile
+    (3000, 2000, 1, [-2999, -700, 0, 511, 512, 1999]),  # one column: tiles of 256..1024 rows, halo edges
+    (700, 900, 1500, [-5, 0, 5]),                      # wider than a tile: the flat kernel
+    (2048, 2048, 3, [-900, 1200]),                     # no diagonal near the tile: no window
+])
+def test_spmm_dia_device_tiling_edges(d, M, K, R, offs):
+    import torch
+    rng = np.random.default_rng(9)
+    V = rng.standard_normal((len(offs), M)) + 1j * rng.standard_normal((len(offs), M))
+    r = np.arange(M)
+    for k, o in enumerate(offs):
+        V[k, ~((r + o >= 0) & (r + o < K))] = 0
+    B = rng.standard_normal((K, R)) + 1j * rng.standard_normal((K, R))
+    ref = O.spmm_dia(offs, V, M, K, B)
+    tV = torch.as_tensor(V, device="cuda").contiguous()
+    tB = torch.as_tensor(B, device="cuda").contiguous()
+    tC = torch.full((M, R), float("nan"), dtype=torch.complex128, device="cuda")
+    off = torch.tensor(offs, dtype=torch.int64, device="cuda")
+    from diaq_amd import capi
+    try:
+        for tiled in (0, 1, 2, 4):
+            capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", tiled))
+            tC.fill_(float("nan"))
+            d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(), True,
+                              torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            assert rel(tC.cpu().numpy(), ref) < 1e-13, tiled
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
