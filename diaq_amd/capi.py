@@ -45,6 +45,7 @@ SIGNATURES = {
     "dq_spmv_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
     "dq_spmv_ntimes_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double), c_int]),
     "dq_spmv_device_banded": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uintptr]),
+    "dq_release_cached_memory": (c_int, []),
     "dq_set_tuning": (c_int, [c_char_p, c_int]),
     "dq_spmv_tune": (c_int, [c_void_p, c_int, c_uintptr, P(ctypes.c_float)]),
     "dq_spmv_tuned_config": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]),

@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "dq_common.hpp"
 
@@ -17,20 +19,116 @@ namespace dq {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Device memory with a cache of freed blocks. A product allocates its result (gigabytes) and its work lists on every
+// call; hipMalloc / hipFree of such blocks cost about a millisecond per product. dev_free waits for the device to be
+// idle -- as hipFree does -- and keeps the block; dev_alloc hands out the smallest cached block of the same device
+// that is large enough and at most a quarter larger (rounded sizes: 2 MiB granularity above 1 MiB), else allocates.
+// The cache holds at most a budget (default an eighth of the device memory, dq_set_tuning "alloc_cache_mb"); beyond
+// it, and when an allocation fails, blocks go back to the runtime. Cached blocks of a process live until it ends
+// (never freed from a static destructor, when the runtime may already be gone), or until dq_release_cached_memory.
+// ------------------------------------------------------------------------------------------------------------------
+namespace {
+struct AllocCache {
+    std::mutex mu;
+    std::unordered_map<void *, std::pair<int, size_t>> live;      // pointer -> (device, capacity)
+    std::map<std::pair<int, size_t>, std::vector<void *>> free_;  // (device, capacity) -> cached blocks
+    size_t cached = 0;
+    int64_t budget = -1;  // bytes; -1: not yet set (an eighth of the device memory)
+};
+AllocCache &cache() {
+    static AllocCache *c = new AllocCache;  // never destroyed: see above
+    return *c;
+}
+size_t round_size(size_t n) {
+    const size_t mib = size_t(1) << 20;
+    if (n >= mib) return (n + 2 * mib - 1) / (2 * mib) * (2 * mib);
+    return (n + 511) / 512 * 512;
+}
+void release_all_locked(AllocCache &c) {
+    (void)hipDeviceSynchronize();
+    for (auto &kv : c.free_)
+        for (void *p : kv.second) (void)hipFree(p);
+    c.free_.clear();
+    c.cached = 0;
+}
+}  // namespace
+
 void *dev_alloc(size_t bytes) {
     if (bytes == 0) return nullptr;
+    AllocCache &c = cache();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const size_t want = round_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.free_.lower_bound({dev, want});
+        if (it != c.free_.end() && it->first.first == dev && it->first.second <= want + want / 4) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            const size_t cap = it->first.second;
+            if (it->second.empty()) c.free_.erase(it);
+            c.cached -= cap;
+            c.live[p] = {dev, cap};
+            return p;
+        }
+    }
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {  // give the cached blocks back and try once more
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            release_all_locked(c);
+        }
+        e = hipMalloc(&p, want);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw AllocError("HIP error: " + std::string(hipGetErrorString(e)) + " allocating " +
                          std::to_string(bytes) + " bytes of device memory");
     }
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.live[p] = {dev, want};
     return p;
 }
 
 void dev_free(void *p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    AllocCache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.live.find(p);
+    if (it == c.live.end()) {  // not ours (cannot happen through DevBuf): plain free
+        (void)hipFree(p);
+        return;
+    }
+    const auto [dev, cap] = it->second;
+    c.live.erase(it);
+    if (c.budget < 0) {
+        size_t fr = 0, tot = 0;
+        c.budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? static_cast<int64_t>(tot / 8) : 0;
+        (void)hipGetLastError();
+    }
+    (void)hipDeviceSynchronize();  // as hipFree: no kernel may still use the block when it is handed out again
+    if (c.cached + cap > static_cast<size_t>(c.budget)) {
+        (void)hipFree(p);
+        return;
+    }
+    c.free_[{dev, cap}].push_back(p);
+    c.cached += cap;
+}
+
+void set_alloc_cache_budget(int64_t mib) {
+    AllocCache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.budget = mib < 0 ? -1 : mib * (int64_t(1) << 20);
+    if (c.budget >= 0 && c.cached > static_cast<size_t>(c.budget)) release_all_locked(c);
+}
+
+void release_cached_memory() {
+    AllocCache &c = cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    release_all_locked(c);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -678,4 +776,8 @@ extern "C" int dq_trace(const dq_devmat *A, double *re, double *im) {
             }
         }
     });
+}
+
+extern "C" int dq_release_cached_memory(void) {
+    return guard([&] { dq::release_cached_memory(); });
 }

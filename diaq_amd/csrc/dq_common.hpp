@@ -77,8 +77,10 @@ inline hipStream_t as_stream(uintptr_t s) { return reinterpret_cast<hipStream_t>
 // ------------------------------------------------------------------------------------------------------------------
 // Device memory
 // ------------------------------------------------------------------------------------------------------------------
-void *dev_alloc(size_t bytes);          // hipMalloc with a clear error; nullptr for 0 bytes
-void dev_free(void *p);                 // hipFree, never throws
+void *dev_alloc(size_t bytes);          // device memory (a cached block or hipMalloc) with a clear error; nullptr for 0
+void dev_free(void *p);                 // back to the cache (after the device is idle) or hipFree; never throws
+void set_alloc_cache_budget(int64_t mib);  // bytes the cache of freed blocks may hold; -1: an eighth of the device
+void release_cached_memory();           // hipFree every cached block
 
 struct DevBuf {
     void *p = nullptr;

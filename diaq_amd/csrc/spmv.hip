@@ -218,6 +218,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "spmv_bands") {
             if (value < 0 || value > 4096) throw ArgError("spmv_bands must be in [0, 4096]");
             g_spmv_bands = value;
+        } else if (k == "alloc_cache_mb") {
+            if (value < -1) throw ArgError("alloc_cache_mb must be -1 (an eighth of the device memory) or >= 0");
+            set_alloc_cache_budget(value);
         } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value)) {
             throw ArgError("unknown tuning key: " + k);
         }

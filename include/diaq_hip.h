@@ -115,6 +115,11 @@ int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontemporal, int *pipelined, int *lds_window,
                          int *xcd_tiles);
 
+/* Return every cached device block to the runtime. The library keeps freed device blocks (up to a budget, by
+ * default an eighth of the device memory; dq_set_tuning "alloc_cache_mb") to serve later allocations without
+ * hipMalloc. No reference counterpart. */
+int dq_release_cached_memory(void);
+
 /* Process-wide tuning knobs for the autotuner and benchmarks: "spmv_rows_per_lane" (0 = automatic, or 1, 2, 4, 8,
  * 16), "spmv_bands" (0 = automatic, or the number of column bands), "spmv_nontemporal" (-1 = automatic, 0, 1:
  * stream the matrix values with non-temporal loads), "spmv_pipelined" (-1 = automatic, 0 = per-row predicated
@@ -125,7 +130,8 @@ int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontempora
  * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
  * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
- * spmm_dia_device kernel. No reference counterpart. */
+ * spmm_dia_device kernel; "alloc_cache_mb" (-1 = an eighth of the device memory, 0 = no cache) bounds the cache of
+ * freed device blocks. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
 /* ------------------------------------------------------------------------------------------------------------------

@@ -552,3 +552,24 @@ def test_spmm_dia_device_tiling_edges(d, M, K, R, offs):
             assert rel(tC.cpu().numpy(), ref) < 1e-13, tiled
     finally:
         capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
+
+
+def test_device_block_cache_reuse(d):
+    """Freed device blocks are cached and handed out again: products computed into reused blocks (holding stale
+    values) equal the first one bit for bit, with the cache on, off, and after releasing it."""
+    from diaq_amd import capi
+    L = capi.lib()
+    rows, cols, offs, vals = Hm.tfim(14, 1.0, 0.7)
+    D = d.to_device(dia_of(d, rows, cols, offs, vals))
+    first = d.product(D, D).to_host()
+    ref = d.to_numpy(first)
+    try:
+        for budget in (-1, 0, 64, -1):
+            capi.check(L.dq_set_tuning(b"alloc_cache_mb", budget))
+            for _ in range(3):
+                C = d.product(D, D)
+                np.testing.assert_array_equal(d.to_numpy(C.to_host()), ref)
+                del C
+            capi.check(L.dq_release_cached_memory())
+    finally:
+        capi.check(L.dq_set_tuning(b"alloc_cache_mb", -1))
