@@ -183,8 +183,15 @@ bool spmm_set_tuning(const std::string &key, int value);
 // Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd", "evo_lds"); false if the key is not one of them.
 bool evo_set_tuning(const std::string &key, int value);
 
-// C = (A B) / divisor with the structural dry-run (product.hip).
-std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);
+// C = (A B) / divisor with the structural dry-run (product.hip). allow_dense: a near-dense product may take the dense
+// path (dense.hip) -- same offsets, values equal up to the summation order; for pow and the matrix exponential.
+std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s,
+                                   bool allow_dense = false);
+// Dense path (dense.hip): scatter to dense, library GEMM on the matrix cores, gather C's diagonals.
+std::unique_ptr<dq_devmat> product_dense(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);
+bool dense_product_pays(const dq_devmat &A, const dq_devmat &B);
+// Dense-path tuning key ("product_dense"); false if it is another key.
+bool dense_set_tuning(const std::string &key, int value);
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);

@@ -233,9 +233,11 @@ __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ i
     }
 }
 
-std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s) {
+std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s,
+                                   bool allow_dense) {
     if (A.cols != B.rows) throw ArgError("diaQMatrix product_cuda: dimension mismatch.");
     if (A.precision != B.precision) throw ArgError("product: operands have different precisions");
+    if (allow_dense && dense_product_pays(A, B)) return product_dense(A, B, divisor, s);
     const DryRun d = dryrun(A.rows, B.cols, A.offs, B.offs);
     // every stored element of C is written by the kernel; only the gaps between diagonals are zeroed
     auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s, false);
@@ -345,16 +347,16 @@ std::unique_ptr<dq_devmat> matrix_pow(const dq_devmat &A, int64_t n, hipStream_t
     std::unique_ptr<dq_devmat> base = copy_devmat(A, s);
     if ((e & (e - 1)) == 0) {
         while (e > 1) {
-            base = product(*base, *base, 1.0, s);
+            base = product(*base, *base, 1.0, s, true);
             e >>= 1;
         }
         return base;
     }
     std::unique_ptr<dq_devmat> result;
     while (e > 0) {
-        if (e & 1ULL) result = result ? product(*result, *base, 1.0, s) : copy_devmat(*base, s);
+        if (e & 1ULL) result = result ? product(*result, *base, 1.0, s, true) : copy_devmat(*base, s);
         e >>= 1ULL;
-        if (e) base = product(*base, *base, 1.0, s);
+        if (e) base = product(*base, *base, 1.0, s, true);
     }
     return result;
 }

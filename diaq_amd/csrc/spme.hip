@@ -68,7 +68,7 @@ static std::unique_ptr<dq_devmat> taylor(const dq_devmat &X, int max_iterations,
     fill_identity(*term, s);
     int i;
     for (i = 1; i <= max_iterations; ++i) {
-        term = product(*term, X, static_cast<double>(i), s);
+        term = product(*term, X, static_cast<double>(i), s, true);
         result = union_add(*result, *term, s);
         if (frobenius(*term, s) < tol) break;
     }

@@ -130,7 +130,8 @@ int dq_release_cached_memory(void);
  * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
  * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
- * spmm_dia_device kernel; "alloc_cache_mb" (-1 = an eighth of the device memory, 0 = no cache) bounds the cache of
+ * spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
+ * exponential goes through a dense library GEMM); "alloc_cache_mb" (-1 = an eighth of the device memory, 0 = no cache) bounds the cache of
  * freed device blocks. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 

@@ -573,3 +573,36 @@ def test_device_block_cache_reuse(d):
             capi.check(L.dq_release_cached_memory())
     finally:
         capi.check(L.dq_set_tuning(b"alloc_cache_mb", -1))
+
+
+def test_dense_path_matches_pair_products(d):
+    """Near-dense products inside pow and expm take the dense path (library GEMM on the matrix cores): the offsets are
+    those of the pair-by-pair products and the values agree to rounding, against the pair path and the oracle."""
+    from diaq_amd import capi
+    L = capi.lib()
+    rng = np.random.default_rng(21)
+    n = 300
+    offs = list(range(-299, 300, 2))  # 300 diagonals: squaring them is near-dense
+    A = banded(n, offs, rng) / 40
+    H = d.from_numpy(A)
+    ref = np.linalg.matrix_power(A, 4)
+    outs = {}
+    try:
+        for dense in (0, 1):
+            capi.check(L.dq_set_tuning(b"product_dense", dense))
+            P = d.pow(H, 4)
+            outs[dense] = (list(P.dIndices), d.to_numpy(P))
+            assert rel(outs[dense][1], ref) < TOL, dense
+        assert outs[0][0] == outs[1][0]
+        np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=0, atol=1e-13 * np.abs(ref).max())
+        # expm of a 10-qubit TFIM: the final squarings are near-dense
+        rows, cols, to, tv = Hm.tfim(10, 1.0, 0.7)
+        Ud, roffs, _, s = O.expm_negative_dense(O.to_dense(O.Dia(rows, cols, to, tv)), to, 0.4)
+        assert s >= 1
+        for dense in (0, 1):
+            capi.check(L.dq_set_tuning(b"product_dense", dense))
+            U = d.expm_negative(dia_of(d, rows, cols, to, tv), 0.4)
+            assert list(U.dIndices) == roffs, dense
+            assert rel(d.to_numpy(U), Ud) < TOL, dense
+    finally:
+        capi.check(L.dq_set_tuning(b"product_dense", 1))
