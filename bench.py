@@ -180,10 +180,17 @@ def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True)
     psi0 = Hm.product_state(n, seed=0)
     dt = 0.05
     norm1 = d.estimate_1norm(H)
+    # one-time schedule selection for this Hamiltonian (a plan step, outside the timed evolution)
+    import ctypes
+    best = ctypes.c_float()
+    capi.check(capi.lib().dq_spmv_tune(H.handle, 3, 0, ctypes.byref(best)))
+    cfg = [ctypes.c_int() for _ in range(5)]
+    capi.check(capi.lib().dq_spmv_tuned_config(H.handle, *[ctypes.byref(c) for c in cfg]))
     tol_step = 1e-6 / steps  # per-step truncation budget for a 1e-6 total
     m_plain = norm_rule_m(norm1 * dt, tol_step)
     out = dict(workload=f"XXZ-{n} (Jxy=1, Jz=0.8, h_i~U[-1,1] seed 0), {steps} steps of dt={dt}, product state seed 0",
-               norm1=norm1, m_plain=m_plain, tol_step=tol_step)
+               norm1=norm1, m_plain=m_plain, tol_step=tol_step,
+               schedule=dict(lds_window=cfg[3].value, xcd_tiles=cfg[4].value))
     sid = torch.cuda.current_stream().cuda_stream
     for label, m in (("plain_fp64", m_plain), ("reference_m100", 100))[: 2 if m100 else 1]:
         pr = torch.as_tensor(psi0.real, device="cuda").contiguous()

@@ -202,7 +202,7 @@ bool lds_window_pays(const dq_devmat &A, int rpt);
 // the tuned choice, else lds_window_pays.
 bool lds_window_for(const dq_devmat &A, int rpt);
 // Row-tile order of row-tile kernels over A (0 = round-robin over the XCDs, 1 = XCD-contiguous): the process-wide knob
-// if set, else the tuned choice, else 0.
+// if set, else the tuned choice, else XCD-contiguous when an eighth of x fits an XCD's L2 and diagonals are near.
 int xcd_tiles_for(const dq_devmat &A);
 
 }  // namespace dq

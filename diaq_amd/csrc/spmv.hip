@@ -147,7 +147,9 @@ bool lds_window_pays(const dq_devmat &A, int rpt) {
 int xcd_tiles_for(const dq_devmat &A) {
     if (g_spmv_xcd >= 0) return g_spmv_xcd;
     if (A.tuned_xcd >= 0) return A.tuned_xcd;
-    return 0;
+    // untuned: XCD-contiguous when an XCD's eighth of x fits its 4 MB L2 and some diagonals are near each other
+    const int64_t x_bytes = static_cast<int64_t>(A.cols) * 2 * elem_bytes(A.precision);
+    return (x_bytes / 8 <= (int64_t(4) << 20) && lds_window_pays(A, 8)) ? 1 : 0;
 }
 
 bool lds_window_for(const dq_devmat &A, int rpt) {
