@@ -730,6 +730,7 @@ extern "C" int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t s
 extern "C" int dq_evo_probe(dq_evo *E, double *est) {
     return guard([&] {
         if (!E || (E->n_jobs && !est)) throw ArgError("dq_evo_probe: bad arguments");
+        // every class's terms first, each class on its own stream (they run concurrently), then the reads
         for (auto &c : E->classes) {
             const int nj = c.jobs_n();
             hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, c.stream, (int64_t)nj * E->n_states * 2,
@@ -741,6 +742,9 @@ extern "C" int dq_evo_probe(dq_evo *E, double *est) {
                 constexpr int NS = decltype(ns)::value;
                 for (int k = 1; k <= kmax; ++k) launch_probe<TV, NS>(c, k, c.stream);
             });
+        }
+        for (auto &c : E->classes) {
+            const int nj = c.jobs_n();
             std::vector<double> h(static_cast<size_t>(nj) * E->n_states * 2);
             DQ_HIP(hipMemcpyAsync(h.data(), c.d_probe.p, h.size() * 8, hipMemcpyDeviceToHost, c.stream));
             DQ_HIP(hipStreamSynchronize(c.stream));
