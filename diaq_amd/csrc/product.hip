@@ -98,13 +98,12 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
     return r;
 }
 
-constexpr int PROD_RPT = 8;
-constexpr int PROD_CHUNK = 256 * PROD_RPT;
 constexpr int PROD_BATCH = 256;
 
 // Process-wide knobs (dq_set_tuning): chunk order across XCDs and non-temporal stores of C.
 static int g_prod_interleave = 1;
 static int g_prod_nt_store = 1;
+static int g_prod_rpt = 8;  // rows per lane: row chunks of 256 * rpt rows
 
 // Item list: the i-th chunk in processing order (row chunk q_of[i]) owns items [first[i], first[i + 1]), result
 // diagonals c_lo[i], c_lo[i] + 1, ...
@@ -138,7 +137,7 @@ __device__ inline int64_t uniform64(int64_t v) {
     return static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo));
 }
 
-template <typename T, bool NTS>
+template <typename T, bool NTS, int PROD_RPT>
 __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ items, const int64_t *__restrict__ xcd_first,
                                                       const int64_t *__restrict__ pair_start,
                                                       const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
@@ -147,6 +146,7 @@ __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ i
                                                       const T *__restrict__ aim, const T *__restrict__ bre,
                                                       const T *__restrict__ bim, T *__restrict__ cre,
                                                       T *__restrict__ cim, double divisor) {
+    constexpr int PROD_CHUNK = 256 * PROD_RPT;
     __shared__ int64_t s_abase[PROD_BATCH], s_bbase[PROD_BATCH];
     __shared__ int32_t s_lo[PROD_BATCH], s_hi[PROD_BATCH];
     // XCD-aware remap: the workgroups of XCD x (x = b mod 8) take the item range [xcd_first[x], xcd_first[x + 1]).
@@ -259,6 +259,8 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     // from the Infinity Cache), or plain ascending order.
     int64_t max_rhi = 0;
     for (int c = 0; c < nc; ++c) max_rhi = std::max(max_rhi, C->meta[c].rhi);
+    const int prod_rpt = g_prod_rpt;
+    const int64_t PROD_CHUNK = 256 * static_cast<int64_t>(prod_rpt);
     const int64_t n_chunks = (max_rhi + PROD_CHUNK - 1) / PROD_CHUNK;
     std::vector<int64_t> order;
     order.reserve(static_cast<size_t>(n_chunks));
@@ -314,11 +316,17 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
                        dim3(256), 0, s, n_chunks, dfirst.as<int64_t>(), dclo.as<int32_t>(), dqof.as<int32_t>(),
                        ditems.as<int2>());
     DQ_LAUNCH_CHECK();
-#define DQ_PROD_LAUNCH(T, NTS)                                                                                     \
-    hipLaunchKernelGGL((product_kernel<T, NTS>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,              \
+#define DQ_PROD_LAUNCH_R(T, NTS, R)                                                                                \
+    hipLaunchKernelGGL((product_kernel<T, NTS, R>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,           \
                        ditems.as<int2>(), dxcd.as<int64_t>(), dps.as<int64_t>(), dsa.as<int32_t>(),               \
                        dsb.as<int32_t>(), A.d_meta.as<DiagMeta>(), B.d_meta.as<DiagMeta>(), C->d_meta.as<DiagMeta>(), \
                        A.re.as<T>(), A.im.as<T>(), B.re.as<T>(), B.im.as<T>(), C->re.as<T>(), C->im.as<T>(), divisor)
+#define DQ_PROD_LAUNCH(T, NTS)                                \
+    do {                                                      \
+        if (prod_rpt == 4) { DQ_PROD_LAUNCH_R(T, NTS, 4); }   \
+        else if (prod_rpt == 16) { DQ_PROD_LAUNCH_R(T, NTS, 16); } \
+        else { DQ_PROD_LAUNCH_R(T, NTS, 8); }                 \
+    } while (0)
     if (A.precision == DQ_F64) {
         if (g_prod_nt_store) DQ_PROD_LAUNCH(double, true);
         else DQ_PROD_LAUNCH(double, false);
@@ -327,6 +335,7 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
         else DQ_PROD_LAUNCH(float, false);
     }
 #undef DQ_PROD_LAUNCH
+#undef DQ_PROD_LAUNCH_R
     DQ_LAUNCH_CHECK();
     DQ_HIP(hipStreamSynchronize(s));  // before the work list is freed
     return C;
@@ -390,6 +399,11 @@ bool product_set_tuning(const std::string &key, int value) {
     if (key == "product_interleave") {
         if (value < 0 || value > 1) throw ArgError("product_interleave must be 0 or 1");
         g_prod_interleave = value;
+        return true;
+    }
+    if (key == "product_rows_per_lane") {
+        if (value != 4 && value != 8 && value != 16) throw ArgError("product_rows_per_lane must be 4, 8 or 16");
+        g_prod_rpt = value;
         return true;
     }
     if (key == "product_nt_store") {

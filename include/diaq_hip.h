@@ -127,7 +127,8 @@ int dq_release_cached_memory(void);
  * diagonals within half a tile) and "spmv_xcd" (-1 = automatic, 0 = row tiles round-robin over the XCDs, 1 = each
  * XCD a contiguous eighth of the tiles) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
- * product's values) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
+ * product's values) and "product_rows_per_lane" (4, 8 = default, 16: row chunks of 256 x that) set the SpGEMM
+ * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
  * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
  * spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
