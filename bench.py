@@ -26,6 +26,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baselines' OpenMP placement (as the reference's runs: one thread per core, close binding); read by the
+# OpenMP runtime when the C oracle first runs, so it is set before anything loads it
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpMV GB/s vs HBM peak; evolutions/s batched-fp32 vs plain fp64 at 1e-6"
@@ -243,9 +247,13 @@ def evolution_cpu_baseline(torch, d, H, offs, vals, rows, psi0, dt, m, steps, si
     from oracle import coracle
     nt = cpu_threads()
     Hp = coracle.packed(offs, vals)
-    t0 = time.perf_counter()
-    cpu = coracle.taylor_step_tasks(rows, Hp, psi0, dt, m, n_threads=nt)
-    t1 = time.perf_counter() - t0
+    coracle.taylor_step_tasks(rows, Hp, psi0, dt, m, n_threads=nt)  # warmup
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        cpu = coracle.taylor_step_tasks(rows, Hp, psi0, dt, m, n_threads=nt)
+        times.append(time.perf_counter() - t0)
+    t1 = float(np.median(times))
     pr = torch.as_tensor(psi0.real, device="cuda").contiguous()
     pi = torch.as_tensor(psi0.imag, device="cuda").contiguous()
     d.hamsim_device(H, pr.data_ptr(), pi.data_ptr(), dt, 1, m, sid)
@@ -256,7 +264,8 @@ def evolution_cpu_baseline(torch, d, H, offs, vals, rows, psi0, dt, m, steps, si
     log(f"[bench] evolution cpu baseline: {t1:.2f} s per step, {per_ev:.1f} s per evolution (x{steps}), {nt} threads, "
         f"GPU-vs-CPU max rel diff {diff:.1e}")
     return dict(value=bytes_per_term * m / t1 / 1e9, unit="GB/s", seconds_per_step=t1, seconds_per_evolution=per_ev,
-                cores=nt, kind="port", sample=f"1 of the {steps} steps (m={m} terms), scaled by {steps}",
+                cores=nt, kind="port",
+                sample=f"1 of the {steps} steps (m={m} terms; median of 3 after 1 warmup), scaled by {steps}",
                 gpu_vs_cpu_max_rel_diff=diff)
 
 
@@ -367,9 +376,13 @@ def spgemm_cpu_baseline(offs, vals, n_rows, C_gpu, nbytes):
     ps[1:] = np.cumsum([len(p) for p in pairs])
     sa = np.array([a for p in pairs for a, _ in p], np.int32)
     sb = np.array([b for p in pairs for _, b in p], np.int32)
-    t0 = time.perf_counter()
-    out = coracle.product_tasks(n_rows, n_rows, n_rows, A, A, (res, ps, sa, sb), n_threads=nt)
-    t = time.perf_counter() - t0
+    coracle.product_tasks(n_rows, n_rows, n_rows, A, A, (res, ps, sa, sb), n_threads=nt)  # warmup
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        out = coracle.product_tasks(n_rows, n_rows, n_rows, A, A, (res, ps, sa, sb), n_threads=nt)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
     worst = 0.0
     for o in (0, 1, res[0]):
         k = list(out[0]).index(o)
@@ -380,7 +393,7 @@ def spgemm_cpu_baseline(offs, vals, n_rows, C_gpu, nbytes):
     log(f"[bench] spgemm cpu baseline: {t:.2f} s ({nbytes / t / 1e9:.1f} GB/s), {nt} threads, GPU-vs-CPU max rel diff "
         f"{worst:.1e}")
     return dict(value=nbytes / t / 1e9, unit="GB/s", seconds=t, cores=nt, kind="port",
-                sample="the full TFIM-20 H*H (same values), one run", gpu_vs_cpu_max_rel_diff=worst)
+                sample="the full TFIM-20 H*H (same values), median of 5 after 1 warmup", gpu_vs_cpu_max_rel_diff=worst)
 
 
 def spmm_section(args, torch, d, capi, Hm):
