@@ -33,6 +33,8 @@ SIGNATURES = {
     "dq_devmat_upload": (c_int, [c_uint32, c_uint32, c_int32, P(c_int32), P(c_int64), P(c_double), P(c_double), c_int,
                                  P(c_void_p)]),
     "dq_devmat_zeros": (c_int, [c_uint32, c_uint32, c_int32, P(c_int32), c_int, P(c_void_p)]),
+    "dq_devmat_from_pauli_sum": (c_int, [c_int, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64), P(c_double),
+                                         P(c_double), c_int, c_uintptr, P(c_void_p)]),
     "dq_devmat_set_packed_device": (c_int, [c_void_p, c_void_p, c_void_p, c_uintptr]),
     "dq_devmat_copy": (c_int, [c_void_p, P(c_void_p)]),
     "dq_devmat_convert": (c_int, [c_void_p, c_int, P(c_void_p)]),

@@ -541,6 +541,21 @@ PYBIND11_MODULE(diaq_gpu, m) {
     }, py::arg("A"), py::arg("precision") = "fp64", "Upload a diaQMatrix to the GPU (a DeviceMatrix).");
     m.def("spMV", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<HostVector> x) { return spmv_dev(*A, *x); },
           py::arg(), py::arg());
+    m.def("pauli_sum_device", [](int n_qubits, std::vector<uint64_t> x_masks, std::vector<uint64_t> z_masks,
+                                 std::vector<std::complex<double>> coefs, const std::string &precision,
+                                 std::uintptr_t stream) {
+        if (x_masks.size() != z_masks.size() || x_masks.size() != coefs.size())
+            throw std::invalid_argument("pauli_sum_device: x_masks, z_masks and coefs must have the same length");
+        std::vector<double> cr(coefs.size()), ci(coefs.size());
+        for (size_t j = 0; j < coefs.size(); ++j) { cr[j] = coefs[j].real(); ci[j] = coefs[j].imag(); }
+        dq_devmat *p = nullptr;
+        check(dq_devmat_from_pauli_sum(n_qubits, static_cast<int>(x_masks.size()), x_masks.data(), z_masks.data(),
+                                       cr.data(), ci.data(), parse_precision(precision), stream, &p));
+        return wrap(p);
+    }, py::arg("n_qubits"), py::arg("x_masks"), py::arg("z_masks"), py::arg("coefs"), py::arg("precision") = "fp64",
+       py::arg("stream") = 0,
+       "H = sum_j coefs[j] P_j built on the device (a DeviceMatrix); term j flips the qubits of x_masks[j] (X or Y) "
+       "and has Z or Y on those of z_masks[j]; qubit k is bit k of the basis index (no reference counterpart)");
     m.def("product", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<DeviceMatrix> B) {
         return product_dev(*A, *B);
     }, py::arg(), py::arg());

@@ -162,3 +162,33 @@ def pauli_sum(n_qubits: int, n_strings: int, rng, min_weight: int = 2, max_weigh
             acc[dv][pos] += c * phase[grp]  # r -> r xor f is one-to-one: no repeated positions
     offs = sorted(d for d, a in acc.items() if np.any(a != 0))
     return N, N, offs, [acc[d] for d in offs], strings
+
+
+def pauli_masks(strings):
+    """(x_masks, z_masks, coefs) of Pauli strings given as (coef, {qubit: 'X' | 'Y' | 'Z'}) pairs (as returned by
+    pauli_sum): x holds the qubits acted on by X or Y, z those acted on by Z or Y."""
+    xs, zs, cs = [], [], []
+    for c, ops in strings:
+        xs.append(sum(1 << q for q, o in ops.items() if o in "XY"))
+        zs.append(sum(1 << q for q, o in ops.items() if o in "YZ"))
+        cs.append(complex(c))
+    return xs, zs, cs
+
+
+def tfim_terms(n_qubits: int, J: float = 1.0, h: float = 0.7):
+    """The transverse-field Ising chain of tfim() as Pauli terms: -J Z_i Z_{i+1} (i < n-1) and -h X_i."""
+    strings = [(-J, {i: "Z", i + 1: "Z"}) for i in range(n_qubits - 1)]
+    strings += [(-h, {i: "X"}) for i in range(n_qubits)]
+    return strings
+
+
+def xxz_terms(n_qubits: int, Jxy: float = 1.0, Jz: float = 0.8, fields=None, seed: int = 0):
+    """The XXZ chain of xxz() as Pauli terms: Jxy (X_i X_{i+1} + Y_i Y_{i+1}) + Jz Z_i Z_{i+1} and h_i Z_i, with the
+    same seeded fields."""
+    if fields is None:
+        fields = np.random.default_rng(seed).uniform(-1.0, 1.0, n_qubits)
+    strings = []
+    for i in range(n_qubits - 1):
+        strings += [(Jxy, {i: "X", i + 1: "X"}), (Jxy, {i: "Y", i + 1: "Y"}), (Jz, {i: "Z", i + 1: "Z"})]
+    strings += [(float(fields[i]), {i: "Z"}) for i in range(n_qubits)]
+    return strings

@@ -54,6 +54,16 @@ int dq_devmat_upload(uint32_t rows, uint32_t cols, int32_t n_diags, const int32_
 int dq_devmat_zeros(uint32_t rows, uint32_t cols, int32_t n_diags, const int32_t *offsets, int precision,
                     dq_devmat **out);
 
+/* A Pauli-sum Hamiltonian H = sum_j c_j P_j on n_qubits qubits (N = 2^n_qubits), built on the device. Term j is
+ * given by x_masks[j] (qubits acted on by X or Y; qubit k = bit k of the basis index), z_masks[j] (qubits acted on by
+ * Z or Y) and the complex coefficient (coef_re[j], coef_im[j]); P|c> = i^{popcount(x & z)} (-1)^{popcount(c & z)}
+ * |c xor x>. The diagonals are the structural set (every +-2^k combination of each term's flipped qubits,
+ * ascending); dq_prune drops the ones whose terms cancel. No reference counterpart: the reference builds matrices on
+ * the host from dense arrays or element by element (src/diaQMatrix.hpp:666-674, include/pythonWrappers.cpp:64). */
+int dq_devmat_from_pauli_sum(int n_qubits, int n_terms, const uint64_t *x_masks, const uint64_t *z_masks,
+                             const double *coef_re, const double *coef_im, int precision, uintptr_t stream,
+                             dq_devmat **out);
+
 /* Fill all values from packed device arrays (diagonal k's len_k values back to back, in slot order) of the
  * matrix precision, on `stream`. No reference counterpart: the bulk constructor for device-generated matrices. */
 int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, const void *im_dev, uintptr_t stream);

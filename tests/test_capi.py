@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from diaq_amd import capi
+from diaq_amd import hamiltonians as Hm
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -137,3 +138,40 @@ def test_binding_host_container_matches_reference_layout():
     np.testing.assert_array_equal(d.to_numpy(I), np.eye(6))
     v = d.from_numpy_vector(np.arange(5) + 1j)
     np.testing.assert_array_equal(d.to_numpy_vector(v), np.arange(5) + 1j)
+
+
+def test_pauli_builder_argument_errors_need_no_gpu():
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    x = np.array([1, 2], np.uint64)
+    z = np.array([0, 8], np.uint64)  # bit 3 beyond 3 qubits
+    cr = np.array([1.0, 0.5])
+    ci = np.zeros(2)
+    args = (capi.ptr(x, ctypes.c_uint64), capi.ptr(z, ctypes.c_uint64), capi.ptr(cr), capi.ptr(ci))
+    rc = L.dq_devmat_from_pauli_sum(3, 2, *args, capi.DQ_F64, 0, ctypes.byref(out))
+    assert rc == capi.DQ_ERR_ARG and b"beyond n_qubits" in L.dq_last_error()
+    rc = L.dq_devmat_from_pauli_sum(0, 2, *args, capi.DQ_F64, 0, ctypes.byref(out))
+    assert rc == capi.DQ_ERR_ARG and b"n_qubits" in L.dq_last_error()
+    rc = L.dq_devmat_from_pauli_sum(3, 2, *args, 7, 0, ctypes.byref(out))
+    assert rc == capi.DQ_ERR_ARG
+
+
+def test_pauli_terms_match_the_chain_builders():
+    """tfim_terms / xxz_terms describe the same matrices as tfim() / xxz() (checked through the numpy Pauli-sum
+    builder's rule on dense matrices)."""
+    n = 5
+    for terms, dia in ((Hm.tfim_terms(n, 1.3, 0.4), Hm.tfim(n, 1.3, 0.4)), (Hm.xxz_terms(n, seed=2), Hm.xxz(n, seed=2))):
+        N = 1 << n
+        dense = np.zeros((N, N), complex)
+        r = np.arange(N)
+        for c, ops in terms:
+            x, z, _ = Hm.pauli_masks([(c, ops)])
+            par = np.bitwise_count(r & z[0]) & 1
+            phase = (1j ** int(np.bitwise_count(np.uint64(x[0] & z[0])))) * (1.0 - 2.0 * par)
+            dense[r ^ x[0], r] += c * phase
+        ref = np.zeros((N, N), complex)
+        for o, v in zip(dia[2], dia[3]):
+            k = np.arange(len(v))
+            rows, cols = (k, k + o) if o >= 0 else (k - o, k)
+            ref[rows, cols] = v
+        np.testing.assert_allclose(dense, ref, atol=1e-14)

@@ -606,3 +606,29 @@ def test_dense_path_matches_pair_products(d):
             assert rel(d.to_numpy(U), Ud) < TOL, dense
     finally:
         capi.check(L.dq_set_tuning(b"product_dense", 1))
+
+
+@pytest.mark.parametrize("kind", ["tfim", "xxz", "random"])
+def test_pauli_sum_device_builder(d, kind):
+    """The device Pauli-sum builder against the host builders: same diagonals once all-zero ones are pruned, same
+    values; the structural set before pruning contains them; FP32 form within FP32 rounding."""
+    n = 9
+    if kind == "tfim":
+        terms = Hm.tfim_terms(n, 1.1, 0.6)
+        _, _, offs, vals = Hm.tfim(n, 1.1, 0.6)
+    elif kind == "xxz":
+        terms = Hm.xxz_terms(n, seed=5)
+        _, _, offs, vals = Hm.xxz(n, seed=5)
+    else:
+        _, _, offs, vals, terms = Hm.pauli_sum(n, 30, np.random.default_rng(6))
+    x, z, c = Hm.pauli_masks(terms)
+    D = d.pauli_sum_device(n, x, z, c)
+    assert set(offs) <= set(D.diagonals) and D.diagonals == sorted(D.diagonals)
+    P = d.prune(D)
+    nonzero = [o for o, v in zip(offs, vals) if np.any(v != 0)]
+    assert P.diagonals == nonzero
+    ref = O.to_dense(O.Dia(1 << n, 1 << n, offs, vals))
+    assert rel(d.to_numpy(P.to_host()), ref) < 1e-15
+    D32 = d.pauli_sum_device(n, x, z, c, precision="fp32")
+    assert D32.precision == "fp32" and D32.diagonals == D.diagonals
+    assert rel(d.to_numpy(d.prune(D32).to_host()), ref) < 1e-6
