@@ -435,6 +435,172 @@ def spmm_section(args, torch, d, capi, Hm):
                 compulsory_bytes=nbytes, gbs=nbytes / (ms * 1e-3) / 1e9, spot_check_rel_err=err)
 
 
+def conversion_section(args, torch, d, capi, Hm):
+    """from_numpy / to_numpy on the device: the 14-qubit TFIM (J = 1, h = 0.7; 29 diagonals) as a dense 16384 x 16384
+    complex128 matrix in HBM (4 GiB) -> DIA (ascending nonzero diagonals) and back. The compulsory bytes of each
+    direction are the dense matrix once (16 N^2) plus the stored DIA values once (16 S)."""
+    r, c, offs, vals = Hm.tfim(14, 1.0, 0.7)
+    N = r
+    T = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+    for o, v in zip(offs, vals):
+        k = torch.arange(len(v), device="cuda")
+        rr, cc = (k, k + o) if o >= 0 else (k - o, k)
+        T[rr, cc] = torch.from_numpy(np.ascontiguousarray(v)).to("cuda")
+    out = torch.empty_like(T)
+    A = d.from_dense_device(T.data_ptr(), N, N)  # warmup
+    d.to_dense_device(A, out.data_ptr())
+    assert A.diagonals == sorted(offs) and torch.equal(out, T)
+    S = sum(len(v) for v in vals)
+    nbytes = 16 * (N * N + S)
+    reps = max(1, min(args.steps, 10))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A = d.from_dense_device(T.data_ptr(), N, N)
+    t_from = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.to_dense_device(A, out.data_ptr())
+    t_to = (time.perf_counter() - t0) / reps
+    log(f"[bench] from_dense_device TFIM-14 dense {N}x{N}: {t_from * 1e3:.3f} ms ({nbytes / t_from / 1e9:.0f} GB/s); "
+        f"to_dense_device {t_to * 1e3:.3f} ms ({nbytes / t_to / 1e9:.0f} GB/s)")
+    res = dict(workload="TFIM-14 (J=1, h=0.7) as dense 16384 x 16384 complex128 <-> DIA (29 diagonals), on device",
+               compulsory_bytes=nbytes, from_dense_ms=t_from * 1e3, from_dense_gbs=nbytes / t_from / 1e9,
+               to_dense_ms=t_to * 1e3, to_dense_gbs=nbytes / t_to / 1e9)
+    # every diagonal stored: a dense random matrix (the tiled kernels); compulsory bytes 16 N^2 each way on each side
+    del A
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    T.copy_(torch.randn((N, N), dtype=torch.complex128, device="cuda", generator=g))
+    A = d.from_dense_device(T.data_ptr(), N, N)
+    d.to_dense_device(A, out.data_ptr())
+    assert A.num_diags == 2 * N - 1 and torch.equal(out, T)
+    del A
+    nb_full = 16 * 2 * N * N
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A = d.from_dense_device(T.data_ptr(), N, N)
+        del A
+    t_from_f = (time.perf_counter() - t0) / reps
+    A = d.from_dense_device(T.data_ptr(), N, N)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.to_dense_device(A, out.data_ptr())
+    t_to_f = (time.perf_counter() - t0) / reps
+    if os.environ.get("DQ_BENCH_CONVERT_TILES"):  # time every tile shape of the tiled kernels
+        for tile in range(4):
+            capi.check(capi.lib().dq_set_tuning(b"dense_convert_tile", tile))
+            B = d.from_dense_device(T.data_ptr(), N, N)
+            del B
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                B = d.from_dense_device(T.data_ptr(), N, N)
+                del B
+            tf = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                d.to_dense_device(A, out.data_ptr())
+            tt = (time.perf_counter() - t0) / reps
+            log(f"[bench]   tile shape {tile}: from_dense {tf * 1e3:.3f} ms, to_dense {tt * 1e3:.3f} ms")
+        capi.check(capi.lib().dq_set_tuning(b"dense_convert_tile", 0))
+    log(f"[bench] from_dense_device dense random {N}x{N}: {t_from_f * 1e3:.3f} ms "
+        f"({(nb_full + 16 * N * N) / t_from_f / 1e9:.0f} GB/s incl. the scan); to_dense_device {t_to_f * 1e3:.3f} ms "
+        f"({nb_full / t_to_f / 1e9:.0f} GB/s)")
+    res["dense_random"] = dict(workload="dense random 16384 x 16384 complex128 <-> DIA (all 32767 diagonals)",
+                               from_dense_ms=t_from_f * 1e3, to_dense_ms=t_to_f * 1e3,
+                               from_dense_gbs=(nb_full + 16 * N * N) / t_from_f / 1e9,
+                               to_dense_gbs=nb_full / t_to_f / 1e9)
+    return res
+
+
+def trotter_gates(d, n, dt, J=1.0, h=0.7):
+    """The gates of one first-order Trotter step of the n-qubit TFIM H = J sum Z_q Z_{q+1} + h sum X_q:
+    exp(-i dt J Z_q Z_{q+1}) (a diagonal 4 x 4 gate on qubits q, q+1) for q = 0..n-2, then exp(-i dt h X_q) (2 x 2)
+    for every qubit, as (device gate, before, after) with qubit q = bit q of the basis index."""
+    zz = np.exp(-1j * dt * J * np.array([1.0, -1.0, -1.0, 1.0]))
+    c, s_ = np.cos(dt * h), np.sin(dt * h)
+    GZ = d.to_device(d.diaQMatrix.from_diagonals([4, 4], [0], [zz]))
+    GX = d.to_device(d.diaQMatrix.from_diagonals([2, 2], [-1, 0, 1], [np.array([-1j * s_]), np.array([c, c]) + 0j,
+                                                                       np.array([-1j * s_])]))
+    return [(GZ, n - 2 - q, q) for q in range(n - 1)] + [(GX, n - 1 - q, q) for q in range(n)]
+
+
+def trotter_section(args, torch, d, capi, Hm, n=28):
+    """First-order Trotter steps of the 28-qubit TFIM (J = 1, h = 0.7, dt = 0.05) on the device: every step applies
+    27 ZZ rotations and 28 X rotations in place (dq_kron_apply_device) to a 2^28-amplitude state (split re/im, FP64;
+    4.3 GB). Algorithmic bytes per gate application: 32 * 2^n (the state read and written once). CPU baseline: the
+    reference's host loop nest (oracle_kron_apply_tasks, y += G x) on a 2^24-amplitude sample of one ZZ and one X
+    gate."""
+    N = 1 << n
+    gates = trotter_gates(d, n, 0.05)
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    vr = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    vi = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    nrm = torch.sqrt((vr * vr + vi * vi).sum())
+    vr /= nrm
+    vi /= nrm
+
+    def step():
+        for G, before, after in gates:
+            d.kron_apply_device(G, before, after, vr.data_ptr(), vi.data_ptr(), vr.data_ptr(), vi.data_ptr(), False,
+                                stream.cuda_stream)
+
+    step()
+    torch.cuda.synchronize()
+    K = max(1, min(args.steps, 5))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(K):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / K
+    norm_err = abs(float(torch.sqrt((vr * vr + vi * vi).sum())) - 1.0)
+    per_gate_ms = ms_step / len(gates)
+    gbs = 32 * N / (per_gate_ms * 1e-3) / 1e9
+    log(f"[bench] trotter TFIM-{n}: {ms_step:.2f} ms per step ({len(gates)} gates, {per_gate_ms:.3f} ms each, "
+        f"{gbs:.0f} GB/s), norm drift {norm_err:.1e}")
+    res = dict(workload=f"first-order Trotter step, TFIM-{n} (J=1, h=0.7, dt=0.05): {n - 1} ZZ + {n} X gates in "
+                        f"place on a 2^{n} state, FP64", ms_per_step=ms_step, gates_per_step=len(gates),
+               ms_per_gate=per_gate_ms, amplitudes_per_s=N / (per_gate_ms * 1e-3), gbs=gbs,
+               roofline={"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+               norm_drift=norm_err)
+    if not args.no_cpu:
+        from oracle import coracle as C
+        from oracle import oracle as O
+        nc = 24
+        Nc = 1 << nc
+        rng = np.random.default_rng(0)
+        xr, xi = rng.standard_normal(Nc), rng.standard_normal(Nc)
+        yr, yi = np.zeros(Nc), np.zeros(Nc)
+        zz = np.diag(np.exp(-1j * 0.05 * np.array([1.0, -1.0, -1.0, 1.0])))
+        c, s_ = np.cos(0.035), np.sin(0.035)
+        rx = np.array([[c, -1j * s_], [-1j * s_, c]])
+        cases = [(4, O.from_dense(zz), nc - 2 - 11, 11), (2, O.from_dense(rx), nc - 1 - 5, 5)]
+        threads = cpu_threads()
+        times = []
+        for gsz, M, before, after in cases:
+            Gp = (gsz,) + C.packed(M.offsets, M.vals)
+            C.kron_apply_tasks(before, Gp, after, xr, xi, yr, yi, threads)  # warm
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                C.kron_apply_tasks(before, Gp, after, xr, xi, yr, yi, threads)
+            times.append((time.perf_counter() - t0) / reps)
+        t_gate = sum(times) / len(times)
+        res["cpu"] = dict(value=round(Nc / t_gate / 1e9, 4), unit="G amplitudes/s per gate", cores=threads,
+                          kind="port", sample=f"one ZZ and one X gate on a 2^{nc}-amplitude state, 3 reps each, "
+                                              "the reference's host loop nest (y += G x)")
+        res["gpu_vs_cpu_amplitude_rate"] = (N / (per_gate_ms * 1e-3)) / (Nc / t_gate)
+        log(f"[bench] trotter CPU baseline: {Nc / t_gate / 1e9:.3f} G amplitudes/s per gate on {threads} threads")
+    return res
+
+
 def expm_section(args, torch, d, capi, Hm):
     """expm(-i H t) at t = 0.1 for the 12-qubit TFIM (J = 1, h = 0.7) by Taylor with scaling and squaring on the GPU,
     against scipy's dense expm."""
@@ -587,7 +753,8 @@ def main():
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
-                    choices=["spmv", "spmv_structured", "spgemm", "spmm_dia", "evolution", "batched"],
+                    choices=["spmv", "spmv_structured", "spgemm", "spmm_dia", "evolution", "batched", "conversion",
+                             "trotter"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -616,6 +783,12 @@ def main():
         elif args.profile == "spmm_dia":
             sm = spmm_section(args, torch, d, capi, Hm)
             log(f"[bench] profile: spmm_dia {sm['ms']:.3f} ms")
+        elif args.profile == "conversion":
+            cv = conversion_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: from_dense {cv['from_dense_ms']:.3f} ms, to_dense {cv['to_dense_ms']:.3f} ms")
+        elif args.profile == "trotter":
+            tr = trotter_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: trotter {tr['ms_per_step']:.2f} ms per step")
         elif args.profile == "batched":
             bt = batched_section(args, torch, d, capi, Hm, quick=True)
             log(f"[bench] profile: batched graph {bt['graph_ms']:.1f} ms")
@@ -639,7 +812,8 @@ def main():
     extra = {}
     if not args.no_evolution:
         for key, fn in (("spmv_structured", structured_spmv_section), ("spgemm", spgemm_section),
-                        ("spmm_dia", spmm_section), ("expm", expm_section)):
+                        ("spmm_dia", spmm_section), ("expm", expm_section), ("conversion", conversion_section),
+                        ("trotter", trotter_section)):
             try:
                 extra[key] = fn(args, torch, d, capi, Hm)
             except Exception as e:
@@ -681,6 +855,8 @@ def main():
         "spgemm": extra.get("spgemm"),
         "spmm_dia": extra.get("spmm_dia"),
         "expm": extra.get("expm"),
+        "conversion": extra.get("conversion"),
+        "trotter": extra.get("trotter"),
         "batched_evolution": bat,
     }
     print(json.dumps(line), flush=True)

@@ -35,7 +35,13 @@ SIGNATURES = {
     "dq_devmat_zeros": (c_int, [c_uint32, c_uint32, c_int32, P(c_int32), c_int, P(c_void_p)]),
     "dq_devmat_from_pauli_sum": (c_int, [c_int, c_int, P(ctypes.c_uint64), P(ctypes.c_uint64), P(c_double),
                                          P(c_double), c_int, c_uintptr, P(c_void_p)]),
+    "dq_devmat_from_dense_device": (c_int, [c_uint32, c_uint32, c_void_p, c_int64, c_int, c_uintptr, P(c_void_p)]),
+    "dq_devmat_to_dense_device": (c_int, [c_void_p, c_void_p, c_int64, c_uintptr]),
     "dq_devmat_set_packed_device": (c_int, [c_void_p, c_void_p, c_void_p, c_uintptr]),
+    "dq_kron_apply_device": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                     c_uintptr]),
+    "dq_kron_apply_host": (c_int, [c_void_p, c_uint32, c_uint32, P(c_double), P(c_double), P(c_double),
+                                   P(c_double)]),
     "dq_devmat_copy": (c_int, [c_void_p, P(c_void_p)]),
     "dq_devmat_convert": (c_int, [c_void_p, c_int, P(c_void_p)]),
     "dq_devmat_download": (c_int, [c_void_p, P(c_int64), P(c_double), P(c_double)]),

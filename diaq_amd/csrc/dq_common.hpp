@@ -157,6 +157,10 @@ namespace dq {
 std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32_t *offsets, int n, int precision,
                                        hipStream_t stream, bool zero_fill = true);
 
+// Zero the padding of M's value arrays (before each diagonal's first stored row and after the last), for a matrix
+// made with zero_fill = false whose stored elements a kernel writes.
+void zero_gaps(dq_devmat &M, hipStream_t s);
+
 // Launch geometry helpers.
 inline unsigned blocks_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
     int64_t b = (work + per_block - 1) / per_block;
@@ -192,6 +196,8 @@ std::unique_ptr<dq_devmat> product_dense(const dq_devmat &A, const dq_devmat &B,
 bool dense_product_pays(const dq_devmat &A, const dq_devmat &B);
 // Dense-path tuning key ("product_dense"); false if it is another key.
 bool dense_set_tuning(const std::string &key, int value);
+// Dense <-> DIA conversion tuning key ("dense_convert"); false if it is another key.
+bool convert_set_tuning(const std::string &key, int value);
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);

@@ -131,6 +131,19 @@ __global__ void zero_gaps_kernel(int n, const DiagMeta *__restrict__ meta, int64
     }
 }
 
+void zero_gaps(dq_devmat &M, hipStream_t s) {
+    const int n = M.n();
+    if (n == 0 || M.total == 0) return;
+    const unsigned g = blocks_for(n + 1, 256, 1 << 20);
+    if (M.precision == DQ_F64)
+        hipLaunchKernelGGL(zero_gaps_kernel<double>, dim3(g), dim3(256), 0, s, n, M.d_meta.as<DiagMeta>(), M.total,
+                           M.re.as<double>(), M.im.as<double>());
+    else
+        hipLaunchKernelGGL(zero_gaps_kernel<float>, dim3(g), dim3(256), 0, s, n, M.d_meta.as<DiagMeta>(), M.total,
+                           M.re.as<float>(), M.im.as<float>());
+    DQ_LAUNCH_CHECK();
+}
+
 __device__ inline int64_t uniform64(int64_t v) {
     const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint64_t>(v) & 0xffffffffu));
     const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint64_t>(v) >> 32));
@@ -243,16 +256,7 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s, false);
     const int nc = C->n();
     if (nc == 0) return C;
-    {
-        const unsigned g = blocks_for(nc + 1, 256, 1 << 20);
-        if (A.precision == DQ_F64)
-            hipLaunchKernelGGL(zero_gaps_kernel<double>, dim3(g), dim3(256), 0, s, nc, C->d_meta.as<DiagMeta>(),
-                               C->total, C->re.as<double>(), C->im.as<double>());
-        else
-            hipLaunchKernelGGL(zero_gaps_kernel<float>, dim3(g), dim3(256), 0, s, nc, C->d_meta.as<DiagMeta>(),
-                               C->total, C->re.as<float>(), C->im.as<float>());
-        DQ_LAUNCH_CHECK();
-    }
+    zero_gaps(*C, s);
     // per row chunk, the contiguous range of result diagonals that meet it (rlo and rhi do not increase with c).
     // Chunk order: the chunks q with q mod 8 == 0 first, then those with q mod 8 == 1, ... (interleave: the XCDs work
     // on neighbouring chunks at the same time, so a far pair's B rows, which are another chunk's near rows, are read

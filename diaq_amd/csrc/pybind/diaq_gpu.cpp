@@ -257,6 +257,19 @@ static void spmv_ntimes_into(const HostDia &A, const HostVector &x, HostVector &
     check(dq_spmv_ntimes_host(D->h.get(), x.re.data(), x.im.data(), y.re.data(), y.im.data(), n));
 }
 
+// y += (I_{2^before} (x) G (x) I_{2^after}) x on host vectors (src/spKron.cpp:286-335, 433-571): the gate is uploaded
+// for the call, like the reference's other host-matrix GPU entry points.
+static std::shared_ptr<HostVector> kron_apply_into(unsigned before, const HostDia &G, unsigned after,
+                                                   const HostVector &x, std::shared_ptr<HostVector> y) {
+    if (G.matrix_shape[0] != G.matrix_shape[1]) throw std::runtime_error("kron times vector: the gate must be square");
+    const unsigned long long n = (unsigned long long)G.matrix_shape[0] << before << after;
+    if (x.numRows != n || y->numRows != n)
+        throw std::runtime_error("kron times vector: dimension mismatch (x and y need 2^before * gate rows * 2^after)");
+    auto D = upload(G);
+    check(dq_kron_apply_host(D->h.get(), before, after, x.re.data(), x.im.data(), y->re.data(), y->im.data()));
+    return y;
+}
+
 static std::shared_ptr<DeviceMatrix> product_dev(const DeviceMatrix &A, const DeviceMatrix &B) {
     dq_devmat *p = nullptr;
     check(dq_product_device(A.h.get(), B.h.get(), &p));
@@ -485,6 +498,28 @@ PYBIND11_MODULE(diaq_gpu, m) {
         spmv_ntimes_into(*A, *x, *y, n);
         return y;
     }, py::arg(), py::arg(), py::arg(), py::arg());
+    m.def("identity_kron_matrix_kron_identity_times_vector",
+          [](unsigned before, std::shared_ptr<HostDia> gate, unsigned after, std::shared_ptr<HostVector> x,
+             std::shared_ptr<HostVector> y) { return kron_apply_into(before, *gate, after, *x, y); },
+          py::arg("before"), py::arg("gate"), py::arg("after"), py::arg("x"), py::arg("y"));
+    m.def("identity_kron_matrix_times_vector",
+          [](unsigned n, std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x, std::shared_ptr<HostVector> y) {
+              return kron_apply_into(n, *A, 0, *x, y);
+          }, py::arg("n"), py::arg("A"), py::arg("x"), py::arg("y"));
+    m.def("matrix_kron_identity_times_vector",
+          [](std::shared_ptr<HostDia> A, unsigned n, std::shared_ptr<HostVector> x, std::shared_ptr<HostVector> y) {
+              return kron_apply_into(0, *A, n, *x, y);
+          }, py::arg("A"), py::arg("n"), py::arg("x"), py::arg("y"));
+    m.def("kron_apply_device", [](std::shared_ptr<DeviceMatrix> G, unsigned before, unsigned after, std::uintptr_t xr,
+                                  std::uintptr_t xi, std::uintptr_t yr, std::uintptr_t yi, bool accumulate,
+                                  std::uintptr_t stream) {
+        check(dq_kron_apply_device(G->h.get(), before, after, reinterpret_cast<const void *>(xr),
+                                   reinterpret_cast<const void *>(xi), reinterpret_cast<void *>(yr),
+                                   reinterpret_cast<void *>(yi), accumulate ? 1 : 0, stream));
+    }, py::arg("G"), py::arg("before"), py::arg("after"), py::arg("xr"), py::arg("xi"), py::arg("yr"), py::arg("yi"),
+       py::arg("accumulate") = true, py::arg("stream") = 0,
+       "y (+)= (I_{2^before} (x) G (x) I_{2^after}) x on device vectors (split re/im, G's precision); x and y may be "
+       "the same vectors for gates up to 16 x 16 (no reference counterpart for device vectors)");
     m.def("pow", [](std::shared_ptr<HostDia> A, long long n) { return pow_host(*A, n); }, py::arg(), py::arg());
     m.def("expm", [](std::shared_ptr<HostDia> A, int max_iterations, double scale_cutoff, double tolerance) {
         return expm_host(*A, max_iterations, scale_cutoff, tolerance);
@@ -556,6 +591,22 @@ PYBIND11_MODULE(diaq_gpu, m) {
        py::arg("stream") = 0,
        "H = sum_j coefs[j] P_j built on the device (a DeviceMatrix); term j flips the qubits of x_masks[j] (X or Y) "
        "and has Z or Y on those of z_masks[j]; qubit k is bit k of the basis index (no reference counterpart)");
+    m.def("from_dense_device", [](std::uintptr_t dense, uint32_t rows, uint32_t cols, int64_t ld,
+                                  const std::string &precision, std::uintptr_t stream) {
+        dq_devmat *p = nullptr;
+        check(dq_devmat_from_dense_device(rows, cols, reinterpret_cast<const double *>(dense), ld < 0 ? cols : ld,
+                                          parse_precision(precision), stream, &p));
+        return wrap(p);
+    }, py::arg("dense"), py::arg("rows"), py::arg("cols"), py::arg("ld") = -1, py::arg("precision") = "fp64",
+       py::arg("stream") = 0,
+       "from_numpy on the device: the nonzero diagonals (ascending) of a row-major complex128 matrix at device "
+       "address dense (row stride ld, default cols), as a DeviceMatrix");
+    m.def("to_dense_device", [](std::shared_ptr<DeviceMatrix> A, std::uintptr_t dense, int64_t ld,
+                                std::uintptr_t stream) {
+        check(dq_devmat_to_dense_device(A->h.get(), reinterpret_cast<double *>(dense),
+                                        ld < 0 ? static_cast<int64_t>(A->matrix_shape()[1]) : ld, stream));
+    }, py::arg("A"), py::arg("dense"), py::arg("ld") = -1, py::arg("stream") = 0,
+       "to_numpy on the device: write A as a row-major complex128 matrix at device address dense (row stride ld)");
     m.def("product", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<DeviceMatrix> B) {
         return product_dev(*A, *B);
     }, py::arg(), py::arg());

@@ -64,6 +64,20 @@ int dq_devmat_from_pauli_sum(int n_qubits, int n_terms, const uint64_t *x_masks,
                              const double *coef_re, const double *coef_im, int precision, uintptr_t stream,
                              dq_devmat **out);
 
+/* from_numpy on the device: the nonzero diagonals of a dense rows x cols matrix that is already in device memory,
+ * in ascending order, as a device matrix. dense is row-major complex128 (re, im interleaved), 16-byte aligned, with
+ * row stride ld >= cols complex elements. A diagonal is kept when one of its elements has a nonzero real or
+ * imaginary part (-0.0 counts as zero). Synchronises the stream once, to read the diagonal set. Replaces from_numpy
+ * for matrices (src/conversions.cpp:103-181, src/conversions_gpu.cu:13-40, src/dense_to_diagonals.hpp:20-58), which
+ * needs the dense array on the host. */
+int dq_devmat_from_dense_device(uint32_t rows, uint32_t cols, const double *dense, int64_t ld, int precision,
+                                uintptr_t stream, dq_devmat **out);
+
+/* to_numpy on the device: write A as a dense row-major complex128 matrix (layout as above; every element of the
+ * rows x cols block is written, zeros off A's diagonals). Replaces to_numpy / convertToDenseMatrix
+ * (src/conversions.cpp:11-48, 70-101), which build the dense array on the host. Returns after the copy is done. */
+int dq_devmat_to_dense_device(const dq_devmat *A, double *dense, int64_t ld, uintptr_t stream);
+
 /* Fill all values from packed device arrays (diagonal k's len_k values back to back, in slot order) of the
  * matrix precision, on `stream`. No reference counterpart: the bulk constructor for device-generated matrices. */
 int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, const void *im_dev, uintptr_t stream);
@@ -142,9 +156,29 @@ int dq_release_cached_memory(void);
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
  * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
  * spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
- * exponential goes through a dense library GEMM); "alloc_cache_mb" (-1 = an eighth of the device memory, 0 = no cache) bounds the cache of
- * freed device blocks. No reference counterpart. */
+ * exponential goes through a dense library GEMM); "dense_convert" (0 = automatic, 1 = per-row kernels, 2 = tiled
+ * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "alloc_cache_mb" (-1 = an
+ * eighth of the device memory, 0 = no cache) bounds the cache of freed device blocks. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
+
+/* ------------------------------------------------------------------------------------------------------------------
+ * Gates on a state vector (Trotter steps) without forming the n-qubit matrix
+ * ---------------------------------------------------------------------------------------------------------------- */
+
+/* y (+)= (I_{2^before} (x) G (x) I_{2^after}) x on device vectors of G's precision (split re/im, length
+ * 2^before * g * 2^after for a square g x g gate G). accumulate = 1 adds into y, as the reference does; 0 overwrites
+ * it. Each output adds G's diagonals in slot order. x and y must not overlap, except that they may be the same
+ * vectors (in place) when g <= 16 and G has at most 64 diagonals. Asynchronous on `stream`. Replaces
+ * identity_kron_matrix_kron_identity_times_vector (src/spKron.cpp:433-496) and its cases before = 0
+ * (matrix_kron_identity_times_vector, src/spKron.cpp:498-571) and after = 0 (identity_kron_matrix_times_vector,
+ * src/spKron.cpp:286-335), which run on the host. */
+int dq_kron_apply_device(const dq_devmat *G, uint32_t before, uint32_t after, const void *xr, const void *xi,
+                         void *yr, void *yi, int accumulate, uintptr_t stream);
+
+/* The same on host FP64 vectors (copied to the device and back; y += ...): the reference's host-vector form
+ * (include/pythonWrappers.cpp:409-455). G must be FP64. */
+int dq_kron_apply_host(const dq_devmat *G, uint32_t before, uint32_t after, const double *xr, const double *xi,
+                       double *yr, double *yi);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * SpGEMM, prune, norms

@@ -40,6 +40,9 @@ def lib():
         L.oracle_taylor_step_tasks.argtypes = [ctypes.c_int64, ctypes.c_int, _i32p, _i64p, _dp, _dp, _dp, _dp,
                                                ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
         L.oracle_taylor_step_tasks.restype = ctypes.c_int
+        L.oracle_kron_apply_tasks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i64p, _dp, _dp,
+                                              ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.oracle_kron_apply_tasks.restype = None
         _LIB = L
     return _LIB
 
@@ -114,3 +117,11 @@ def taylor_step_tasks(rows, H, psi, dt, m, n_threads=0):
     if rc != 0:
         raise MemoryError("oracle_taylor_step_tasks: allocation failed")
     return outr + 1j * outi
+
+
+def kron_apply_tasks(before, G, after, xr, xi, yr, yi, n_threads=0):
+    """y += (I (x) G (x) I) x in place on split float64 arrays with the reference's CPU loop nest (diaq_oracle.c:
+    oracle_kron_apply_tasks). G is (g, offsets, starts, re, im) as from ``packed``."""
+    g, offs, starts, gre, gim = G
+    lib().oracle_kron_apply_tasks(before, g, len(offs), _p(offs, _i32p), _p(starts, _i64p), _p(gre, _dp),
+                                  _p(gim, _dp), after, _p(xr, _dp), _p(xi, _dp), _p(yr, _dp), _p(yi, _dp), n_threads)

@@ -288,3 +288,32 @@ int oracle_taylor_step_tasks(int64_t rows, int n_diags, const int32_t *offsets, 
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * y += (I_{2^before} (x) G (x) I_{2^after}) x with the reference's CPU loop nest
+ * (src/spKron.cpp:433-496, identity_kron_matrix_kron_identity_times_vector): parallel over (block rep, column j)
+ * pairs (omp collapse(2)), then per diagonal slot of the square g x g gate a strided run of len elements.
+ * G's diagonals are packed: slot k holds len_k = g - |d_k| values from starts[k].
+ * ------------------------------------------------------------------------------------------------------------- */
+void oracle_kron_apply_tasks(int before, int g, int n_diags, const int32_t *offsets, const int64_t *starts,
+                             const double *gre, const double *gim, int after, const double *xr, const double *xi,
+                             double *yr, double *yi, int n_threads) {
+    const int64_t B = (int64_t)1 << before, A = (int64_t)1 << after;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel for collapse(2) schedule(static) num_threads(n_threads)
+    for (int64_t rep = 0; rep < B; ++rep) {
+        for (int64_t j = 0; j < A; ++j) {
+            for (int k = 0; k < n_diags; ++k) {
+                const int64_t d = offsets[k], ad = d < 0 ? -d : d, len = g - ad;
+                const double *vr = gre + starts[k], *vi = gim + starts[k];
+                const int64_t skip = rep * (A * (len + ad));
+                const int64_t xo = d < 0 ? skip : A * d + skip, yo = d < 0 ? skip + A * ad : skip;
+                for (int64_t i = 0; i < len; ++i) {
+                    const int64_t xa = xo + i * A + j, ya = yo + i * A + j;
+                    yr[ya] += vr[i] * xr[xa] - vi[i] * xi[xa];
+                    yi[ya] += vr[i] * xi[xa] + vi[i] * xr[xa];
+                }
+            }
+        }
+    }
+}

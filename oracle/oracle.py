@@ -475,6 +475,41 @@ def spmm_dia(offsets, vals, n_rows: int, n_cols: int, B: np.ndarray) -> np.ndarr
     return C
 
 
+# ----------------------------------------------------------------------------------------------------------------------
+# Gates on a state vector without forming the n-qubit matrix
+# ----------------------------------------------------------------------------------------------------------------------
+def kron_apply(before: int, G: Dia, after: int, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """y + (I_{2^before} (x) G (x) I_{2^after}) x, following identity_kron_matrix_kron_identity_times_vector
+    (src/spKron.cpp:433-496): for every block rep < 2^before and column j < 2^after, each diagonal slot in order adds
+    G's values into y at the reference's offsets (skip_places = rep * 2^after * (len + |d|); x_offset and y_offset
+    shifted by 2^after * |d| on the side the diagonal leaves). G is square. Returns the new y (y is not modified)."""
+    if G.rows != G.cols:
+        raise ValueError("gate must be square")
+    B, A = 1 << before, 1 << after
+    x = np.asarray(x, np.complex128)
+    y = np.array(y, np.complex128)
+    for rep in range(B):
+        for a, d in enumerate(G.offsets):
+            ln = diagonal_length(d, G.rows, G.cols)
+            skip = rep * (A * (ln + abs(d)))
+            x_off = skip if d < 0 else A * d + skip
+            y_off = skip - A * d if d < 0 else skip
+            i = np.arange(ln)[:, None] * A + np.arange(A)[None, :]  # (i, j)
+            y[y_off + i] += G.vals[a][:, None] * x[x_off + i]
+    return y
+
+
+def identity_kron_matrix_times_vector(n: int, G: Dia, x, y):
+    """y + (I_{2^n} (x) G) x (src/spKron.cpp:286-335): the case after = 0."""
+    return kron_apply(n, G, 0, x, y)
+
+
+def matrix_kron_identity_times_vector(G: Dia, n: int, x, y):
+    """y + (G (x) I_{2^n}) x (src/spKron.cpp:498-571; zero values are skipped there, which adds nothing): the case
+    before = 0."""
+    return kron_apply(0, G, n, x, y)
+
+
 def dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize=16, index_bytes=8):
     """Storage and diagonal budgets for DIA x dense (src/dia_viability.hpp:42-63)."""
     stored = float(n_diags) * float(n_rows)
@@ -562,6 +597,8 @@ PINS = {
     "expm/expm_negative": "scipy.linalg.expm at <= 12 qubits; TFIM-12 s = 1",
     "taylor_action/hamsim": "scipy.linalg.expm_multiply / dense expm; 2-norm preservation",
     "spmm_dia": "dense matmul",
+    "kron_apply": "np.kron(np.kron(I, G), I) @ x on random gates (the reference's test_api_vs_numpy.py check); "
+                  "the spKron.cpp:2231-2262 prime-gate case against the kron matrix",
     "dia_viability": "hand-computed budget cases",
     "mlp_loss_grad": "central finite differences of mlp_forward's loss",
 }

@@ -175,3 +175,16 @@ def test_pauli_terms_match_the_chain_builders():
             rows, cols = (k, k + o) if o >= 0 else (k - o, k)
             ref[rows, cols] = v
         np.testing.assert_allclose(dense, ref, atol=1e-14)
+
+
+def test_dense_conversion_argument_errors_need_no_gpu():
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    assert L.dq_devmat_from_dense_device(4, 4, None, 4, capi.DQ_F64, 0, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    assert b"dense is NULL" in L.dq_last_error()
+    assert L.dq_devmat_from_dense_device(4, 4, 4096, 3, capi.DQ_F64, 0, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    assert b"ld must be" in L.dq_last_error()
+    assert L.dq_devmat_from_dense_device(4, 4, 4104, 4, capi.DQ_F64, 0, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    assert b"aligned" in L.dq_last_error()
+    assert L.dq_devmat_from_dense_device(4, 4, 4096, 4, 9, 0, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    assert L.dq_devmat_to_dense_device(None, 4096, 4, 0) == capi.DQ_ERR_ARG

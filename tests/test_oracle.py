@@ -435,3 +435,54 @@ def test_mlp_gradient_vs_finite_differences():
     h = 1e-6
     fd = np.array([(L(p + h * e) - L(p - h * e)) / (2 * h) for e in np.eye(n)])
     np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+
+
+def test_kron_apply_vs_numpy_kron():
+    """The reference's own check (python_tests/test_api_vs_numpy.py:61-80): random square gates of size 2..11,
+    before/after in 0..2, against np.kron(np.kron(I, G), I) @ x; y accumulates."""
+    rng = np.random.default_rng(12)
+    for _ in range(25):
+        g = int(rng.integers(2, 12))
+        b, a = (int(v) for v in rng.integers(0, 3, size=2))
+        D = (rng.standard_normal((g, g)) + 1j * rng.standard_normal((g, g))) * (rng.random((g, g)) < 0.6)
+        G = O.from_dense(D)
+        big = np.kron(np.kron(np.eye(1 << b), D), np.eye(1 << a))
+        x = rng.standard_normal(big.shape[1]) + 1j * rng.standard_normal(big.shape[1])
+        y0 = rng.standard_normal(big.shape[0]) + 1j * rng.standard_normal(big.shape[0])
+        np.testing.assert_allclose(O.kron_apply(b, G, a, x, y0), y0 + big @ x, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(O.identity_kron_matrix_times_vector(b, G, x[:g << b], np.zeros(g << b)),
+                                   np.kron(np.eye(1 << b), D) @ x[:g << b], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(O.matrix_kron_identity_times_vector(G, a, x[:g << a], np.zeros(g << a)),
+                                   np.kron(D, np.eye(1 << a)) @ x[:g << a], rtol=0, atol=1e-12)
+
+
+def test_kron_apply_prime_gate_case():
+    """cpp_tests/test_main.cpp:2231-2262: the 4 x 4 prime gate, (G (x) I_4) ones(16) -- each output is its row sum."""
+    P = np.array([[2, 3, 5, 7], [11, 13, 17, 19], [23, 29, 31, 37], [41, 43, 47, 53]], complex)
+    y = O.matrix_kron_identity_times_vector(O.from_dense(P), 2, np.ones(16, complex), np.zeros(16, complex))
+    np.testing.assert_array_equal(y, np.repeat(P.sum(axis=1), 4))
+    # a Hadamard on the first of 4 qubits applied to |0000> (test_main.cpp:2291-2330)
+    H = np.array([[1, 1], [1, -1]], complex) / np.sqrt(2)
+    x = np.zeros(16, complex)
+    x[0] = 1
+    y = O.matrix_kron_identity_times_vector(O.from_dense(H), 3, x, np.zeros(16, complex))
+    ref = np.zeros(16, complex)
+    ref[0] = ref[8] = 1 / np.sqrt(2)
+    np.testing.assert_allclose(y, ref, atol=1e-15)
+
+
+def test_c_kron_apply_tasks_matches_numpy_oracle():
+    """The C restatement of the reference's Kronecker loop (the CPU baseline) against oracle.kron_apply."""
+    from oracle import coracle as C
+    rng = np.random.default_rng(21)
+    for g, b, a in ((2, 3, 4), (4, 0, 5), (5, 2, 0), (8, 1, 3), (11, 2, 2)):
+        D = (rng.standard_normal((g, g)) + 1j * rng.standard_normal((g, g))) * (rng.random((g, g)) < 0.6)
+        M = O.from_dense(D)
+        n = g << (b + a)
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        y0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        offs, starts, re, im = C.packed(M.offsets, M.vals)
+        yr, yi = y0.real.copy(), y0.imag.copy()
+        C.kron_apply_tasks(b, (g, offs, starts, re, im), a, np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag),
+                           yr, yi)
+        np.testing.assert_allclose(yr + 1j * yi, O.kron_apply(b, M, a, x, y0), rtol=0, atol=1e-12)
