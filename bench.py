@@ -549,26 +549,37 @@ def trotter_section(args, torch, d, capi, Hm, n=28):
             d.kron_apply_device(G, before, after, vr.data_ptr(), vi.data_ptr(), vr.data_ptr(), vi.data_ptr(), False,
                                 stream.cuda_stream)
 
-    step()
-    torch.cuda.synchronize()
+    def timed(fn, K):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(K):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / K
+
     K = max(1, min(args.steps, 5))
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(K):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_step = e0.elapsed_time(e1) / K
-    norm_err = abs(float(torch.sqrt((vr * vr + vi * vi).sum())) - 1.0)
+    ms_step = timed(step, K)  # one gate per pass
     per_gate_ms = ms_step / len(gates)
     gbs = 32 * N / (per_gate_ms * 1e-3) / 1e9
-    log(f"[bench] trotter TFIM-{n}: {ms_step:.2f} ms per step ({len(gates)} gates, {per_gate_ms:.3f} ms each, "
-        f"{gbs:.0f} GB/s), norm drift {norm_err:.1e}")
+    plan = d.GatePlan(n, [(G, after) for G, before, after in gates])
+    ms_fused = timed(lambda: plan.run(vr.data_ptr(), vi.data_ptr(), 1, stream.cuda_stream), K)
+    per_pass_ms = ms_fused / plan.n_passes
+    gbs_fused = 32 * N / (per_pass_ms * 1e-3) / 1e9
+    norm_err = abs(float(torch.sqrt((vr * vr + vi * vi).sum())) - 1.0)
+    log(f"[bench] trotter TFIM-{n}: gate by gate {ms_step:.2f} ms per step ({len(gates)} gates, {per_gate_ms:.3f} ms "
+        f"each, {gbs:.0f} GB/s); fused plan {ms_fused:.2f} ms per step ({plan.n_passes} passes, {per_pass_ms:.3f} ms "
+        f"each, {gbs_fused:.0f} GB/s); norm drift {norm_err:.1e}")
     res = dict(workload=f"first-order Trotter step, TFIM-{n} (J=1, h=0.7, dt=0.05): {n - 1} ZZ + {n} X gates in "
-                        f"place on a 2^{n} state, FP64", ms_per_step=ms_step, gates_per_step=len(gates),
-               ms_per_gate=per_gate_ms, amplitudes_per_s=N / (per_gate_ms * 1e-3), gbs=gbs,
-               roofline={"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                        f"place on a 2^{n} state, FP64", gates_per_step=len(gates),
+               steps_per_s=1e3 / ms_fused, ms_per_step=ms_fused, passes_per_step=plan.n_passes,
+               ms_per_pass=per_pass_ms, gbs_per_pass=gbs_fused,
+               roofline={"bound": "hbm", "achieved": round(gbs_fused, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs_fused / HBM_PEAK_GBS, 4), "kernel": "gates_pass_kernel"},
+               gate_by_gate=dict(ms_per_step=ms_step, ms_per_gate=per_gate_ms, gbs=gbs,
+                                 amplitudes_per_s=N / (per_gate_ms * 1e-3)),
                norm_drift=norm_err)
     if not args.no_cpu:
         from oracle import coracle as C
@@ -596,7 +607,8 @@ def trotter_section(args, torch, d, capi, Hm, n=28):
         res["cpu"] = dict(value=round(Nc / t_gate / 1e9, 4), unit="G amplitudes/s per gate", cores=threads,
                           kind="port", sample=f"one ZZ and one X gate on a 2^{nc}-amplitude state, 3 reps each, "
                                               "the reference's host loop nest (y += G x)")
-        res["gpu_vs_cpu_amplitude_rate"] = (N / (per_gate_ms * 1e-3)) / (Nc / t_gate)
+        res["cpu"]["trotter_steps_per_s_at_2^28"] = Nc / t_gate / N / len(gates)
+        res["gpu_vs_cpu_steps_per_s"] = (1e3 / ms_fused) / (Nc / t_gate / N / len(gates))
         log(f"[bench] trotter CPU baseline: {Nc / t_gate / 1e9:.3f} G amplitudes/s per gate on {threads} threads")
     return res
 
@@ -788,7 +800,7 @@ def main():
             log(f"[bench] profile: from_dense {cv['from_dense_ms']:.3f} ms, to_dense {cv['to_dense_ms']:.3f} ms")
         elif args.profile == "trotter":
             tr = trotter_section(args, torch, d, capi, Hm)
-            log(f"[bench] profile: trotter {tr['ms_per_step']:.2f} ms per step")
+            log(f"[bench] profile: trotter {tr['ms_per_step']:.2f} ms per step (fused plan)")
         elif args.profile == "batched":
             bt = batched_section(args, torch, d, capi, Hm, quick=True)
             log(f"[bench] profile: batched graph {bt['graph_ms']:.1f} ms")

@@ -42,6 +42,10 @@ SIGNATURES = {
                                      c_uintptr]),
     "dq_kron_apply_host": (c_int, [c_void_p, c_uint32, c_uint32, P(c_double), P(c_double), P(c_double),
                                    P(c_double)]),
+    "dq_gate_plan_create": (c_int, [c_int, c_int, P(c_void_p), P(c_uint32), P(c_void_p)]),
+    "dq_gate_plan_passes": (c_int, [c_void_p, P(c_int)]),
+    "dq_gate_plan_run": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_uintptr]),
+    "dq_gate_plan_destroy": (None, [c_void_p]),
     "dq_devmat_copy": (c_int, [c_void_p, P(c_void_p)]),
     "dq_devmat_convert": (c_int, [c_void_p, c_int, P(c_void_p)]),
     "dq_devmat_download": (c_int, [c_void_p, P(c_int64), P(c_double), P(c_double)]),
@@ -51,6 +55,8 @@ SIGNATURES = {
     "dq_devmat_offsets": (c_int, [c_void_p, P(c_int32)]),
     "dq_spmv_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_uintptr]),
     "dq_spmv_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
+    "dq_spmv_ntimes_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                      c_uintptr]),
     "dq_spmv_ntimes_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double), c_int]),
     "dq_spmv_device_banded": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uintptr]),
     "dq_release_cached_memory": (c_int, []),
@@ -75,6 +81,7 @@ SIGNATURES = {
     "dq_spmm_dia_device": (c_int, [c_uintptr, c_uintptr, c_int, c_int64, c_int64, c_uintptr, c_int64, c_uintptr, c_int,
                                    c_uintptr]),
     "dq_dia_viability": (c_int, [c_int64, c_int64, c_int64, c_int64, c_int64, c_int, c_int, P(DiaViability)]),
+    "dq_dia_viability_mi355x": (c_int, [c_int64, c_int64, c_int64, c_int64, c_int64, c_int, c_int, P(DiaViability)]),
     "dq_evo_create": (c_int, [c_int, P(c_void_p), c_int, c_int, P(c_void_p)]),
     "dq_evo_destroy": (None, [c_void_p]),
     "dq_evo_info": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),

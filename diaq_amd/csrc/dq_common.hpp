@@ -198,6 +198,8 @@ bool dense_product_pays(const dq_devmat &A, const dq_devmat &B);
 bool dense_set_tuning(const std::string &key, int value);
 // Dense <-> DIA conversion tuning key ("dense_convert"); false if it is another key.
 bool convert_set_tuning(const std::string &key, int value);
+// Gate-pass tuning key ("gates_tile_qubits"); false if it is another key.
+bool gates_set_tuning(const std::string &key, int value);
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);

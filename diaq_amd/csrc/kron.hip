@@ -12,35 +12,13 @@
 #include <algorithm>
 #include <cmath>
 
-#include "dia_rows.hpp"
 #include "dq_common.hpp"
+#include "kron_core.hpp"
 
 namespace dq {
 
 constexpr int KRON_REG_MAX = 16;     // largest gate of the register path
 constexpr int KRON_MAX_SLOTS = 64;   // slots staged in LDS on the register path
-
-// The terms of one diagonal with offset D for every row r of the column: static register indexes only.
-template <typename T, int GM, int D>
-__device__ inline void kron_diag_terms(int g, int rlo, int rhi, const T *vr, const T *vi, const T (&xr)[GM],
-                                       const T (&xi)[GM], Acc<T> (&ar)[GM], Acc<T> (&ai)[GM]) {
-#pragma unroll
-    for (int r = 0; r < GM; ++r) {
-        if (r + D >= 0 && r + D < GM) {
-            if (r >= rlo && r < rhi && r < g && r + D < g) cmac(ar[r], ai[r], vr[r], vi[r], xr[r + D], xi[r + D]);
-        }
-    }
-}
-
-// Dispatch a runtime offset d in [-(GM - 1), GM - 1] to kron_diag_terms<D>.
-template <typename T, int GM, int D>
-__device__ inline void kron_diag_dispatch(int d, int g, int rlo, int rhi, const T *vr, const T *vi,
-                                          const T (&xr)[GM], const T (&xi)[GM], Acc<T> (&ar)[GM], Acc<T> (&ai)[GM]) {
-    if constexpr (D < GM) {
-        if (d == D) kron_diag_terms<T, GM, D>(g, rlo, rhi, vr, vi, xr, xi, ar, ai);
-        else kron_diag_dispatch<T, GM, D + 1>(d, g, rlo, rhi, vr, vi, xr, xi, ar, ai);
-    }
-}
 
 template <typename T, int GM>
 __global__ __launch_bounds__(256) void kron_apply_reg_kernel(int64_t ncols, int after, int g, int nd,

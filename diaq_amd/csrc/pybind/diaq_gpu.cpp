@@ -257,6 +257,31 @@ static void spmv_ntimes_into(const HostDia &A, const HostVector &x, HostVector &
     check(dq_spmv_ntimes_host(D->h.get(), x.re.data(), x.im.data(), y.re.data(), y.im.data(), n));
 }
 
+// A planned gate sequence (dq_gate_plan): keeps its gates alive for as long as the plan lives.
+struct GatePlan {
+    std::shared_ptr<dq_gate_plan> h;
+    std::vector<std::shared_ptr<DeviceMatrix>> gates;
+    int n_qubits = 0;
+    GatePlan(int n, const std::vector<std::pair<std::shared_ptr<DeviceMatrix>, unsigned>> &seq) : n_qubits(n) {
+        std::vector<const dq_devmat *> g;
+        std::vector<uint32_t> q;
+        for (const auto &e : seq) {
+            if (!e.first) throw std::invalid_argument("GatePlan: None gate");
+            gates.push_back(e.first);
+            g.push_back(e.first->h.get());
+            q.push_back(e.second);
+        }
+        dq_gate_plan *p = nullptr;
+        check(dq_gate_plan_create(n, static_cast<int>(g.size()), g.data(), q.data(), &p));
+        h.reset(p, dq_gate_plan_destroy);
+    }
+    int passes() const {
+        int n = 0;
+        check(dq_gate_plan_passes(h.get(), &n));
+        return n;
+    }
+};
+
 // y += (I_{2^before} (x) G (x) I_{2^after}) x on host vectors (src/spKron.cpp:286-335, 433-571): the gate is uploaded
 // for the call, like the reference's other host-matrix GPU entry points.
 static std::shared_ptr<HostVector> kron_apply_into(unsigned before, const HostDia &G, unsigned after,
@@ -477,6 +502,24 @@ PYBIND11_MODULE(diaq_gpu, m) {
        py::arg("itemsize") = 16, py::arg("index_bytes") = 8);
 
     // kernels on host matrices (each call uploads, as the reference's host API does)
+    m.def("dia_viability_mi355x", [](int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
+                              int itemsize, int index_bytes) {
+        dq_dia_viability_t v;
+        check(dq_dia_viability_mi355x(n_rows, n_cols, n_diags, nnz, n_right, itemsize, index_bytes, &v));
+        py::dict d;
+        d["viable"] = (bool)v.viable;
+        d["storage_ok"] = (bool)v.storage_ok;
+        d["diagonals_ok"] = (bool)v.diagonals_ok;
+        d["dia_bytes"] = v.dia_bytes;
+        d["csr_bytes"] = v.csr_bytes;
+        d["density"] = v.density;
+        d["diagonal_budget"] = v.diagonal_budget;
+        d["reason"] = std::string(v.reason);
+        return d;
+    }, py::arg("n_rows"), py::arg("n_cols"), py::arg("n_diags"), py::arg("nnz"), py::arg("n_right"),
+       py::arg("itemsize") = 16, py::arg("index_bytes") = 8);
+
+    // kernels on host matrices (each call uploads, as the reference's host API does)
     m.def("product", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostDia> B) { return product_host(*A, *B); });
     m.def("trace", [](std::shared_ptr<HostDia> A) { return trace_host(*A); });
     m.def("spMV", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x) {
@@ -510,6 +553,15 @@ PYBIND11_MODULE(diaq_gpu, m) {
           [](std::shared_ptr<HostDia> A, unsigned n, std::shared_ptr<HostVector> x, std::shared_ptr<HostVector> y) {
               return kron_apply_into(0, *A, n, *x, y);
           }, py::arg("A"), py::arg("n"), py::arg("x"), py::arg("y"));
+    m.def("spmv_ntimes_device", [](std::shared_ptr<DeviceMatrix> A, std::uintptr_t xr, std::uintptr_t xi,
+                                   std::uintptr_t yr, std::uintptr_t yi, int num_times, std::uintptr_t work_r,
+                                   std::uintptr_t work_i, std::uintptr_t stream) {
+        check(dq_spmv_ntimes_device(A->h.get(), reinterpret_cast<const void *>(xr), reinterpret_cast<const void *>(xi),
+                                    reinterpret_cast<void *>(yr), reinterpret_cast<void *>(yi), num_times,
+                                    reinterpret_cast<void *>(work_r), reinterpret_cast<void *>(work_i), stream));
+    }, py::arg("A"), py::arg("xr"), py::arg("xi"), py::arg("yr"), py::arg("yi"), py::arg("num_times"),
+       py::arg("work_r") = 0, py::arg("work_i") = 0, py::arg("stream") = 0,
+       "y = A^num_times x on device vectors (x == y allowed with work vectors); spMV_ntimes on resident data");
     m.def("kron_apply_device", [](std::shared_ptr<DeviceMatrix> G, unsigned before, unsigned after, std::uintptr_t xr,
                                   std::uintptr_t xi, std::uintptr_t yr, std::uintptr_t yi, bool accumulate,
                                   std::uintptr_t stream) {
@@ -529,6 +581,20 @@ PYBIND11_MODULE(diaq_gpu, m) {
         return expm_negative_host(*A, t, max_iterations, scale_cutoff, tolerance);
     }, py::arg("A"), py::arg("t"), py::arg("max_iterations") = 100, py::arg("scale_cutoff") = 1.0,
        py::arg("tolerance") = 1e-12);
+
+    py::class_<GatePlan, std::shared_ptr<GatePlan>>(m, "GatePlan",
+        "A gate sequence on an n-qubit state, planned into fused passes: GatePlan(n_qubits, [(gate, first_qubit), ...]) "
+        "with 2^k x 2^k DeviceMatrix gates (k <= 4) acting on qubits first_qubit .. first_qubit + k - 1 "
+        "(no reference counterpart)")
+        .def(py::init<int, const std::vector<std::pair<std::shared_ptr<DeviceMatrix>, unsigned>> &>(),
+             py::arg("n_qubits"), py::arg("gates"))
+        .def_property_readonly("n_passes", &GatePlan::passes)
+        .def_readonly("n_qubits", &GatePlan::n_qubits)
+        .def("run", [](const GatePlan &P, std::uintptr_t re, std::uintptr_t im, int repetitions, std::uintptr_t stream) {
+            check(dq_gate_plan_run(P.h.get(), reinterpret_cast<void *>(re), reinterpret_cast<void *>(im), repetitions,
+                                   stream));
+        }, py::arg("re"), py::arg("im"), py::arg("repetitions") = 1, py::arg("stream") = 0,
+           "state <- (G_last ... G_first)^repetitions state in place on split re/im device vectors");
 
     // device-resident matrices
     py::class_<DeviceMatrix, std::shared_ptr<DeviceMatrix>>(m, "DeviceMatrix",

@@ -14,6 +14,7 @@
 // products are added, so a lane keeps 16 loads in flight, and a diagonal's offset and base pointers are uniform
 // (scalar) values.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -230,32 +231,61 @@ extern "C" int dq_spmm_dia_device(uintptr_t offsets, uintptr_t vals, int n_diags
     });
 }
 
-// Storage and diagonal budgets (src/dia_viability.hpp:36-63): DIA bytes <= 0.9 x CSR bytes, and
-// n_diags <= max(16, n_rows / 32), capped at 8 when n_right >= 32.
+// Storage and diagonal budgets (src/dia_viability.hpp:36-63) with a set of constants: DIA bytes <= max_ratio x CSR
+// bytes, and n_diags <= max(min_budget, n_rows / rows_per_diagonal), capped at wide_budget when n_right >= wide_right.
+struct ViabilityRule {
+    double max_ratio;
+    int64_t min_budget, rows_per_diagonal, wide_right, wide_budget;
+};
+
+// The reference's constants (an A100 sweep) and the ones fitted on MI355X by diaq_amd.viability_sweep over the same
+// 360-case grid: no case passing them ran slower in DIA than hipSPARSE's CSR x dense, and they keep 196 of the 293
+// cases where DIA was faster (the reference's constants keep 63).
+constexpr ViabilityRule REFERENCE_RULE{0.9, 16, 32, 32, 8};
+constexpr ViabilityRule MI355X_RULE{6.0, 64, 128, 16, 16};
+
+static dq_dia_viability_t viability(const ViabilityRule &R, int64_t n_rows, int64_t n_diags, int64_t nnz,
+                                    int64_t n_right, int itemsize, int index_bytes) {
+    dq_dia_viability_t v;
+    std::memset(&v, 0, sizeof(v));
+    const double stored = static_cast<double>(n_diags) * static_cast<double>(n_rows);
+    v.dia_bytes = stored * itemsize + static_cast<double>(n_diags) * 8.0;
+    v.csr_bytes = static_cast<double>(nnz) * (itemsize + index_bytes) + static_cast<double>(n_rows + 1) * index_bytes;
+    v.density = stored > 0 ? static_cast<double>(nnz) / stored : 0.0;
+    v.storage_ok = n_diags > 0 && v.dia_bytes <= R.max_ratio * v.csr_bytes;
+    v.diagonal_budget = std::max<int64_t>(R.min_budget, n_rows / R.rows_per_diagonal);
+    if (n_right >= R.wide_right) v.diagonal_budget = std::min<int64_t>(v.diagonal_budget, R.wide_budget);
+    v.diagonals_ok = n_diags <= v.diagonal_budget;
+    v.viable = v.storage_ok && v.diagonals_ok;
+    std::string reason;
+    if (!v.storage_ok) {
+        char pct[32];
+        std::snprintf(pct, sizeof(pct), "%g%%", R.max_ratio * 100.0);
+        reason = std::string("storage: DIA would take more than ") + pct +
+                 " of CSR's bytes (diagonals too sparsely filled)";
+    } else if (!v.diagonals_ok) {
+        reason = "diagonals: " + std::to_string(n_diags) + " over the budget of " + std::to_string(v.diagonal_budget);
+    } else {
+        reason = "viable";
+    }
+    std::strncpy(v.reason, reason.c_str(), sizeof(v.reason) - 1);
+    return v;
+}
+
 extern "C" int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
                                 int itemsize, int index_bytes, dq_dia_viability_t *out) {
     return guard([&] {
         if (!out) throw ArgError("out is NULL");
         (void)n_cols;
-        dq_dia_viability_t v;
-        std::memset(&v, 0, sizeof(v));
-        const double stored = static_cast<double>(n_diags) * static_cast<double>(n_rows);
-        v.dia_bytes = stored * itemsize + static_cast<double>(n_diags) * 8.0;
-        v.csr_bytes = static_cast<double>(nnz) * (itemsize + index_bytes) + static_cast<double>(n_rows + 1) * index_bytes;
-        v.density = stored > 0 ? static_cast<double>(nnz) / stored : 0.0;
-        v.storage_ok = n_diags > 0 && v.dia_bytes <= 0.9 * v.csr_bytes;
-        v.diagonal_budget = std::max<int64_t>(16, n_rows / 32);
-        if (n_right >= 32) v.diagonal_budget = std::min<int64_t>(v.diagonal_budget, 8);
-        v.diagonals_ok = n_diags <= v.diagonal_budget;
-        v.viable = v.storage_ok && v.diagonals_ok;
-        std::string reason;
-        if (!v.storage_ok)
-            reason = "storage: DIA would take more than 90% of CSR's bytes (diagonals too sparsely filled)";
-        else if (!v.diagonals_ok)
-            reason = "diagonals: " + std::to_string(n_diags) + " over the budget of " + std::to_string(v.diagonal_budget);
-        else
-            reason = "viable";
-        std::strncpy(v.reason, reason.c_str(), sizeof(v.reason) - 1);
-        *out = v;
+        *out = viability(REFERENCE_RULE, n_rows, n_diags, nnz, n_right, itemsize, index_bytes);
+    });
+}
+
+extern "C" int dq_dia_viability_mi355x(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz,
+                                       int64_t n_right, int itemsize, int index_bytes, dq_dia_viability_t *out) {
+    return guard([&] {
+        if (!out) throw ArgError("out is NULL");
+        (void)n_cols;
+        *out = viability(MI355X_RULE, n_rows, n_diags, nnz, n_right, itemsize, index_bytes);
     });
 }

@@ -224,7 +224,7 @@ extern "C" int dq_set_tuning(const char *key, int value) {
             if (value < -1) throw ArgError("alloc_cache_mb must be -1 (an eighth of the device memory) or >= 0");
             set_alloc_cache_budget(value);
         } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value) &&
-                   !dense_set_tuning(k, value) && !convert_set_tuning(k, value)) {
+                   !dense_set_tuning(k, value) && !convert_set_tuning(k, value) && !gates_set_tuning(k, value)) {
             throw ArgError("unknown tuning key: " + k);
         }
     });
@@ -367,6 +367,46 @@ extern "C" int dq_spmv_host(const dq_devmat *A, const double *xr, const double *
         spmv(*A, x.r.p, x.i.p, y.r.p, y.i.p, 0);
         DQ_LAUNCH_CHECK();
         y.get(yr, yi);
+    });
+}
+
+extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int n,
+                                     void *work_r, void *work_i, uintptr_t stream) {
+    return guard([&] {
+        if (!A) throw ArgError("matrix is NULL");
+        if (n < 0) throw ArgError("spmv_ntimes_cuda: num_times must be >= 0");
+        if (!xr || !xi || !yr || !yi) throw ArgError("spmv_ntimes_device: NULL vector");
+        const int64_t M = A->rows, N = A->cols;
+        const size_t eb = elem_bytes(A->precision);
+        hipStream_t s = as_stream(stream);
+        if (n == 0) {
+            if (M != N)
+                throw ArgError("spmv_ntimes_cuda: num_times==0 requires square matrix (M==N) to copy x -> y");
+            if (xr != yr) DQ_HIP(hipMemcpyAsync(yr, xr, M * eb, hipMemcpyDeviceToDevice, s));
+            if (xi != yi) DQ_HIP(hipMemcpyAsync(yi, xi, M * eb, hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        if (n > 1 && M != N)
+            throw ArgError("spmv_ntimes_cuda: repeated SpMV requires square matrix (M==N) for num_times > 1");
+        const bool in_place = xr == yr || xi == yi;
+        if (in_place && (xr != yr || xi != yi)) throw ArgError("spmv_ntimes_device: x and y overlap in one part only");
+        if ((n > 1 || in_place) && (!work_r || !work_i))
+            throw ArgError("spmv_ntimes_device: work vectors are needed for num_times > 1 or x == y");
+        // destinations alternate between y and the work vectors so that the last product lands in y; with x == y and
+        // an odd count the first product must not overwrite x, so the order flips and a copy ends the sequence
+        const bool flip = in_place && (n % 2 == 1);
+        const void *sr = xr, *si = xi;
+        for (int t = 0; t < n; ++t) {
+            const bool to_y = (((n - 1 - t) % 2) == 0) != flip;
+            void *dr = to_y ? yr : work_r, *di = to_y ? yi : work_i;
+            spmv(*A, sr, si, dr, di, s);
+            sr = dr;
+            si = di;
+        }
+        if (flip) {
+            DQ_HIP(hipMemcpyAsync(yr, work_r, M * eb, hipMemcpyDeviceToDevice, s));
+            DQ_HIP(hipMemcpyAsync(yi, work_i, M * eb, hipMemcpyDeviceToDevice, s));
+        }
     });
 }
 

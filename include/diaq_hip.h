@@ -124,6 +124,13 @@ int dq_spmv_host(const dq_devmat *A, const double *xr, const double *xi, double 
 /* y = A^n x for host vectors, by n SpMVs on the device. Replaces spMV_ntimes_cuda (src/spMV_ntimes_gpu.cu:61-263). */
 int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi, int n);
 
+/* y = A^num_times x on device vectors of A's precision, asynchronous on `stream`: the reference's spMV_ntimes
+ * (src/spMV_ntimes_gpu.cu:61-263, same errors and num_times == 0 / no-diagonal rules) on data that stays resident.
+ * work_r / work_i are one more vector of A.rows elements, needed when num_times > 1 or x == y (in place: the result
+ * overwrites x); the products alternate between y and the work vectors. */
+int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int num_times,
+                          void *work_r, void *work_i, uintptr_t stream);
+
 /* y = A x with the column range split into n_bands passes (n_bands = 0 picks the library's choice for the size);
  * same result as dq_spmv_device up to summation order. No reference counterpart (tuning entry point). */
 int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi_dev, void *yr_dev, void *yi_dev,
@@ -157,7 +164,9 @@ int dq_release_cached_memory(void);
  * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
  * spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
  * exponential goes through a dense library GEMM); "dense_convert" (0 = automatic, 1 = per-row kernels, 2 = tiled
- * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "alloc_cache_mb" (-1 = an
+ * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "dense_convert_tile" (0 =
+ * 32 x 64 tiles, the default, 1 = 64 x 32, 2 = 64 x 64, 3 = 128 x 32) their tiled kernels' tile; "gates_tile_qubits"
+ * (10, 11 = default, 12) the tile of later gate plans; "alloc_cache_mb" (-1 = an
  * eighth of the device memory, 0 = no cache) bounds the cache of freed device blocks. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 
@@ -179,6 +188,25 @@ int dq_kron_apply_device(const dq_devmat *G, uint32_t before, uint32_t after, co
  * (include/pythonWrappers.cpp:409-455). G must be FP64. */
 int dq_kron_apply_host(const dq_devmat *G, uint32_t before, uint32_t after, const double *xr, const double *xi,
                        double *yr, double *yi);
+
+/* A planned sequence of gates (a Trotter step): gate j (a 2^k x 2^k device matrix, 1 <= k <= 4) acts on qubits
+ * first_qubit[j] .. first_qubit[j] + k - 1 of an n_qubits state (qubit q = bit q of the amplitude index), i.e. it is
+ * I_{2^(n - q - k)} (x) G_j (x) I_{2^q}. The plan cuts the sequence into passes that each stream the state through
+ * HBM once and apply a stretch of gates to tiles of 2^11 amplitudes held in LDS (dq_set_tuning
+ * "gates_tile_qubits": 10, 11 or 12). Running it is bitwise equal to applying the gates in order with
+ * dq_kron_apply_device(G_j, n - q - k, q, state, state, accumulate 0). All gates share one precision; the plan copies
+ * their values (at most 512 nonzero entries per gate). n_qubits must be at least the tile qubits and at most 40.
+ * No reference counterpart: the reference applies each gate in its own host loop (src/spKron.cpp:433-496), the step
+ * that the diagonal-budgeted Trotterization workflow of the reference's README (README.md:340-352) builds on. */
+typedef struct dq_gate_plan dq_gate_plan;
+int dq_gate_plan_create(int n_qubits, int n_gates, const dq_devmat *const *gates, const uint32_t *first_qubit,
+                        dq_gate_plan **out);
+/* Number of passes over the state per run of the plan. */
+int dq_gate_plan_passes(const dq_gate_plan *plan, int *n_passes);
+/* state <- (G_{m-1} ... G_0)^repetitions state, in place on the split re/im device vectors (2^n_qubits amplitudes,
+ * the gates' precision); asynchronous on `stream`. */
+int dq_gate_plan_run(const dq_gate_plan *plan, void *re, void *im, int repetitions, uintptr_t stream);
+void dq_gate_plan_destroy(dq_gate_plan *plan);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * SpGEMM, prune, norms
@@ -263,6 +291,13 @@ typedef struct {
 
 int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right, int itemsize,
                      int index_bytes, dq_dia_viability_t *out);
+
+/* The same two budgets with constants fitted on MI355X over the reference's 360-case grid (spmm_dia_device against
+ * hipSPARSE's CSR x dense, complex double): DIA bytes <= 6 x CSR bytes, n_diags <= max(64, n_rows / 128), capped at 16
+ * when n_right >= 16. No case passing them ran slower in DIA; they keep 196 of the 293 DIA wins (the reference's
+ * constants keep 63). No reference counterpart (the reference's constants come from an A100). */
+int dq_dia_viability_mi355x(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
+                            int itemsize, int index_bytes, dq_dia_viability_t *out);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * Batched time evolution (no reference counterpart: the reference evolves one state of one Hamiltonian per call,

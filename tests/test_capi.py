@@ -188,3 +188,17 @@ def test_dense_conversion_argument_errors_need_no_gpu():
     assert b"aligned" in L.dq_last_error()
     assert L.dq_devmat_from_dense_device(4, 4, 4096, 4, 9, 0, ctypes.byref(out)) == capi.DQ_ERR_ARG
     assert L.dq_devmat_to_dense_device(None, 4096, 4, 0) == capi.DQ_ERR_ARG
+
+
+def test_gate_plan_argument_errors_need_no_gpu():
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    q = (ctypes.c_uint32 * 1)(0)
+    assert L.dq_gate_plan_create(12, 0, None, q, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    gates = (ctypes.c_void_p * 1)(None)
+    assert L.dq_gate_plan_create(12, 1, gates, q, ctypes.byref(out)) == capi.DQ_ERR_ARG
+    assert b"NULL gate" in L.dq_last_error()
+    assert L.dq_set_tuning(b"gates_tile_qubits", 13) == capi.DQ_ERR_ARG
+    n = ctypes.c_int()
+    assert L.dq_gate_plan_passes(None, ctypes.byref(n)) == capi.DQ_ERR_ARG
+    assert L.dq_gate_plan_run(None, None, None, 1, 0) == capi.DQ_ERR_ARG

@@ -726,3 +726,38 @@ def dense_round_trip_case(d):
     torch.cuda.synchronize()
     ref = (T @ x).cpu().numpy()
     assert rel((yr + 1j * yi).cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("n_times", [0, 1, 2, 3, 6])
+def test_spmv_ntimes_device_resident_and_in_place(d, n_times):
+    """spMV_ntimes on resident device vectors: the same bits as the host-vector entry point (which runs the same
+    kernels), within 1e-12 of the oracle; in place (x == y) for even and odd counts; the num_times rules."""
+    import torch
+    rng = np.random.default_rng(40 + n_times)
+    n = 3000
+    A = banded(n, [0, 1, -1, 7, -200], rng) * 0.4
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dA = d.to_device(d.from_numpy(A))
+    ref = O.spmv_ntimes(O.from_dense(A), x, n_times)
+    host = d.to_numpy_vector(d.spMV_ntimes(d.from_numpy(A), d.from_numpy_vector(x), n_times))
+    xr = torch.tensor(x.real, device="cuda")
+    xi = torch.tensor(x.imag, device="cuda")
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    wr, wi = torch.empty_like(xr), torch.empty_like(xi)
+    d.spmv_ntimes_device(dA, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times, wr.data_ptr(),
+                         wi.data_ptr())
+    torch.cuda.synchronize()
+    got = yr.cpu().numpy() + 1j * yi.cpu().numpy()
+    assert np.array_equal(got, host)
+    assert rel(got, ref) < TOL
+    d.spmv_ntimes_device(dA, xr.data_ptr(), xi.data_ptr(), xr.data_ptr(), xi.data_ptr(), n_times, wr.data_ptr(),
+                         wi.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(xr.cpu().numpy() + 1j * xi.cpu().numpy(), host)
+    if n_times > 1:
+        with pytest.raises(RuntimeError, match="work vectors"):
+            d.spmv_ntimes_device(dA, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times)
+    R = d.to_device(d.from_numpy(np.ones((3, 4), complex)))
+    if n_times == 0:
+        with pytest.raises(RuntimeError, match="requires square"):
+            d.spmv_ntimes_device(R, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 0)
