@@ -515,6 +515,35 @@ def conversion_section(args, torch, d, capi, Hm):
     return res
 
 
+def builder_section(args, torch, d, capi, Hm):
+    """Hamiltonians built on the device (dq_devmat_from_pauli_sum): the 24-qubit XXZ chain with random fields
+    (seed 1; 93 Pauli terms, 47 structural diagonals of 2^24 rows) and the 24-qubit TFIM, against the host chain
+    builders plus upload (the reference's route: host arrays, then a DeviceMatrix)."""
+    res = {}
+    for name, terms, host in (("XXZ-24", Hm.xxz_terms(24, seed=1), lambda: Hm.xxz(24, seed=1)),
+                              ("TFIM-24", Hm.tfim_terms(24, 1.0, 0.7), lambda: Hm.tfim(24, 1.0, 0.7))):
+        x, z, c = Hm.pauli_masks(terms)
+        D = d.pauli_sum_device(24, x, z, c)  # warm
+        del D
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        D = d.pauli_sum_device(24, x, z, c)
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+        nb = D.stored * 16
+        t0 = time.perf_counter()
+        r, cc, offs, vals = host()
+        H = d.to_device(d.diaQMatrix.from_diagonals([r, cc], offs, vals))
+        torch.cuda.synchronize()
+        t_host = time.perf_counter() - t0
+        log(f"[bench] pauli_sum_device {name}: {len(terms)} terms, {D.num_diags} diagonals, {t_dev * 1e3:.2f} ms "
+            f"({nb / t_dev / 1e9:.0f} GB/s of DIA written); host build + upload {t_host * 1e3:.0f} ms")
+        res[name] = dict(terms=len(terms), diagonals=D.num_diags, device_ms=t_dev * 1e3, host_build_upload_ms=t_host * 1e3,
+                         dia_bytes=nb)
+        del D, H
+    return res
+
+
 def trotter_gates(d, n, dt, J=1.0, h=0.7):
     """The gates of one first-order Trotter step of the n-qubit TFIM H = J sum Z_q Z_{q+1} + h sum X_q:
     exp(-i dt J Z_q Z_{q+1}) (a diagonal 4 x 4 gate on qubits q, q+1) for q = 0..n-2, then exp(-i dt h X_q) (2 x 2)
@@ -529,10 +558,10 @@ def trotter_gates(d, n, dt, J=1.0, h=0.7):
 
 def trotter_section(args, torch, d, capi, Hm, n=28):
     """First-order Trotter steps of the 28-qubit TFIM (J = 1, h = 0.7, dt = 0.05) on the device: every step applies
-    27 ZZ rotations and 28 X rotations in place (dq_kron_apply_device) to a 2^28-amplitude state (split re/im, FP64;
-    4.3 GB). Algorithmic bytes per gate application: 32 * 2^n (the state read and written once). CPU baseline: the
-    reference's host loop nest (oracle_kron_apply_tasks, y += G x) on a 2^24-amplitude sample of one ZZ and one X
-    gate."""
+    27 ZZ rotations and 28 X rotations in place to a 2^28-amplitude state (split re/im, FP64; 4.3 GB), once gate by
+    gate (dq_kron_apply_device) and once as a fused plan (dq_gate_plan: 5 passes per step). Algorithmic bytes per
+    pass over the state: 32 * 2^n (read and written once). CPU baseline: the reference's host loop nest
+    (oracle_kron_apply_tasks, y += G x) on a 2^24-amplitude sample of one ZZ and one X gate."""
     N = 1 << n
     gates = trotter_gates(d, n, 0.05)
     stream = torch.cuda.current_stream()
@@ -766,7 +795,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
                     choices=["spmv", "spmv_structured", "spgemm", "spmm_dia", "evolution", "batched", "conversion",
-                             "trotter"],
+                             "trotter", "builders"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -801,6 +830,8 @@ def main():
         elif args.profile == "trotter":
             tr = trotter_section(args, torch, d, capi, Hm)
             log(f"[bench] profile: trotter {tr['ms_per_step']:.2f} ms per step (fused plan)")
+        elif args.profile == "builders":
+            builder_section(args, torch, d, capi, Hm)
         elif args.profile == "batched":
             bt = batched_section(args, torch, d, capi, Hm, quick=True)
             log(f"[bench] profile: batched graph {bt['graph_ms']:.1f} ms")
@@ -825,7 +856,7 @@ def main():
     if not args.no_evolution:
         for key, fn in (("spmv_structured", structured_spmv_section), ("spgemm", spgemm_section),
                         ("spmm_dia", spmm_section), ("expm", expm_section), ("conversion", conversion_section),
-                        ("trotter", trotter_section)):
+                        ("trotter", trotter_section), ("builders", builder_section)):
             try:
                 extra[key] = fn(args, torch, d, capi, Hm)
             except Exception as e:
@@ -869,6 +900,7 @@ def main():
         "expm": extra.get("expm"),
         "conversion": extra.get("conversion"),
         "trotter": extra.get("trotter"),
+        "builders": extra.get("builders"),
         "batched_evolution": bat,
     }
     print(json.dumps(line), flush=True)
