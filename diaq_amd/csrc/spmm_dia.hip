@@ -6,7 +6,9 @@
 // Layout (as the reference): vals[k * n_rows + r] = A[r, r + offsets[k]]; B is n_cols x n_right and C n_rows x
 // n_right, both row-major; double or interleaved complex double.
 //
-// Mapping on gfx950: lane e owns C's flat element e = r * n_right + j. Its B operand for diagonal k is
+// Default mapping on gfx950 (spmm_dia_cols_kernel): a lane owns two adjacent columns of one row of C when n_right is
+// even, one element otherwise. One element per lane (spmm_dia_kernel): lane e owns C's flat element
+// e = r * n_right + j. Its B operand for diagonal k is
 // B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run of B and
 // writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles rows,
 // which coalesces only for wide B). The diagonal values vals[k * n_rows + r] are shared by the n_right lanes of a row.
@@ -29,9 +31,10 @@ struct cplx {
     double x, y;
 };
 
+// acc + a b with one fixed rounding sequence (explicit fmas), so every kernel variant gives the same bits.
 __device__ inline double fma_val(double acc, double a, double b) { return fma(a, b, acc); }
 __device__ inline cplx fma_val(cplx acc, cplx a, cplx b) {
-    return cplx{acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
+    return cplx{fma(-a.y, b.y, fma(a.x, b.x, acc.x)), fma(a.y, b.x, fma(a.x, b.y, acc.y))};
 }
 // The diagonal values are read once and C is written once: non-temporal, so B's rows keep the caches.
 typedef double d2v __attribute__((ext_vector_type(2)));  // one 16-byte access for a complex value
@@ -86,6 +89,57 @@ __global__ __launch_bounds__(256) void spmm_dia_kernel(I n_rows, I n_cols, I n_r
                 if (ok[u]) acc = fma_val(acc, a[u], b[u]);
         }
         st_stream(C + e, acc);
+    }
+}
+
+// Row-chunk form: lane w owns row r = w / (n_right / J) and the J consecutive columns [jc J, jc J + J) of C, with
+// jc = w mod (n_right / J) (J divides n_right). Per diagonal it reads the row's value once and J consecutive B values
+// of one row of B (contiguous 16-byte loads), instead of one value and one B element per lane: a wave moves the same
+// B bytes with J times fewer value loads. U diagonals' loads are issued before their products are added in order.
+template <typename T, typename I, int J, int U>
+__global__ __launch_bounds__(256) void spmm_dia_cols_kernel(I n_rows, I n_cols, I n_right, int n_diags,
+                                                            const int64_t *__restrict__ offsets,
+                                                            const T *__restrict__ vals, const T *__restrict__ B,
+                                                            T *__restrict__ C) {
+    const I nchunk = n_right / J;
+    const I total = n_rows * nchunk;
+    for (I w = static_cast<I>(blockIdx.x) * blockDim.x + threadIdx.x; w < total;
+         w += static_cast<I>(gridDim.x) * blockDim.x) {
+        const I r = w / nchunk;
+        const int64_t e0 = static_cast<int64_t>(r) * n_right + static_cast<int64_t>(w - r * nchunk) * J;
+        T acc[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) acc[j] = zero_val<T>();
+        for (int k0 = 0; k0 < n_diags; k0 += U) {
+            T a[U], b[U][J];
+            bool ok[U];
+            int64_t o[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) o[u] = offsets[k0 + u < n_diags ? k0 + u : n_diags - 1];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + u;
+                const int64_t c = static_cast<int64_t>(r) + o[u];
+                ok[u] = k < n_diags && c >= 0 && c < static_cast<int64_t>(n_cols);
+                a[u] = zero_val<T>();
+#pragma unroll
+                for (int j = 0; j < J; ++j) b[u][j] = zero_val<T>();
+                if (ok[u]) {
+                    a[u] = ld_stream(vals + static_cast<int64_t>(k) * n_rows + r);
+                    const T *bp = B + e0 + o[u] * static_cast<int64_t>(n_right);
+#pragma unroll
+                    for (int j = 0; j < J; ++j) b[u][j] = bp[j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (ok[u]) {
+#pragma unroll
+                    for (int j = 0; j < J; ++j) acc[j] = fma_val(acc[j], a[u], b[u][j]);
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) st_stream(C + e0 + j, acc[j]);
     }
 }
 
@@ -159,12 +213,43 @@ __global__ __launch_bounds__(256) void spmm_dia_tiled_kernel(I n_rows, I n_cols,
     }
 }
 
+template <typename T, int J>
+static void launch_cols(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off,
+                        const T *vals, const T *B, T *C, bool small, hipStream_t s) {
+    constexpr int U = J >= 8 ? 4 : 8;
+    const unsigned grid = blocks_for(n_rows * (n_right / J), 256, 1 << 20);
+    if (small)
+        hipLaunchKernelGGL((spmm_dia_cols_kernel<T, uint32_t, J, U>), dim3(grid), dim3(256), 0, s, (uint32_t)n_rows,
+                           (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C);
+    else
+        hipLaunchKernelGGL((spmm_dia_cols_kernel<T, int64_t, J, U>), dim3(grid), dim3(256), 0, s, n_rows, n_cols,
+                           n_right, n_diags, off, vals, B, C);
+}
+
+// dq_set_tuning "spmm_cols_per_lane": 0 = automatic (2 when n_right is even, else 1), 1, 2, 4, 8 = that many columns
+// per lane when it divides n_right, -1 = one C element per lane (spmm_dia_kernel). Two columns per lane measured
+// fastest for every even n_right tried (8, 16, 32, 128: 10-22 % over one element per lane); 4 and 8 were slower (a
+// wave's loads then spread over 4-8 times more cache lines per instruction).
+static int g_spmm_cols = 0;
+
 template <typename T>
 static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off, const T *vals,
                    const T *B, T *C, int ept, hipStream_t s) {
     const int64_t total = n_rows * n_right;
     if (total == 0) return;
     const bool small = total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31);
+    if (ept == 0 && g_spmm_cols >= 0) {
+        int J = 1;
+        if (g_spmm_cols == 0) J = n_right % 2 == 0 ? 2 : 1;
+        else if (n_right % g_spmm_cols == 0) J = g_spmm_cols;
+        if (J == 8) launch_cols<T, 8>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        else if (J == 4) launch_cols<T, 4>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        else if (J == 2) launch_cols<T, 2>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        else launch_cols<T, 1>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) throw HipError(std::string("spmm_dia_device: ") + hipGetErrorString(err));
+        return;
+    }
     if (ept > 0 && n_right <= 256 * ept) {
         const int64_t tr = 256 * ept / n_right;
         const int64_t tiles = (n_rows + tr - 1) / tr;
@@ -201,6 +286,12 @@ static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags,
 static int g_spmm_tiled = 0;
 
 bool spmm_set_tuning(const std::string &key, int value) {
+    if (key == "spmm_cols_per_lane") {
+        if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+            throw ArgError("spmm_cols_per_lane must be -1, 0, 1, 2, 4 or 8");
+        g_spmm_cols = value;
+        return true;
+    }
     if (key != "spmm_tiled") return false;
     if (value != 0 && value != 1 && value != 2 && value != 4) throw ArgError("spmm_tiled must be 0, 1, 2 or 4");
     g_spmm_tiled = value;

@@ -161,8 +161,9 @@ int dq_release_cached_memory(void);
  * product's values) and "product_rows_per_lane" (4, 8 = default, 16: row chunks of 256 x that) set the SpGEMM
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
- * "spmm_tiled" (0 = flat kernel, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) picks the
- * spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
+ * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
+ * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)
+ * pick the spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
  * exponential goes through a dense library GEMM); "dense_convert" (0 = automatic, 1 = per-row kernels, 2 = tiled
  * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "dense_convert_tile" (0 =
  * 32 x 64 tiles, the default, 1 = 64 x 32, 2 = 64 x 64, 3 = 128 x 32) their tiled kernels' tile; "gates_tile_qubits"

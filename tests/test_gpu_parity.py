@@ -509,17 +509,35 @@ def test_spmm_dia_device(d, case, complex_):
     tB = torch.as_tensor(B, device="cuda").to(dt).contiguous()
     tC = torch.empty((M, R), dtype=dt, device="cuda")
     off = torch.tensor(offs, dtype=torch.int64, device="cuda")
+    spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, complex_, ref)
+
+
+# (spmm_tiled, spmm_cols_per_lane): the row-chunk kernel (automatic and each width), the one-element-per-lane
+# kernel, the LDS-tiled kernel with 1, 2, 4 elements per lane
+SPMM_KERNELS = [(0, 0), (0, -1), (0, 1), (0, 2), (0, 4), (0, 8), (1, 0), (2, 0), (4, 0)]
+
+
+def spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, complex_, ref):
+    """Every spmm_dia kernel: within 1e-13 of the oracle and the same bits as each other."""
     from diaq_amd import capi
+    first = None
     try:
-        for tiled in (0, 1, 2, 4):  # flat kernel, LDS-tiled kernel with 1, 2, 4 elements per lane
+        for tiled, cols in SPMM_KERNELS:
             capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", tiled))
+            capi.check(capi.lib().dq_set_tuning(b"spmm_cols_per_lane", cols))
             tC.fill_(float("nan"))
             d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(),
                               complex_, torch.cuda.current_stream().cuda_stream)
             torch.cuda.synchronize()
-            assert rel(tC.cpu().numpy(), ref) < 1e-13, tiled
+            got = tC.cpu().numpy()
+            assert rel(got, ref) < 1e-13, (tiled, cols)
+            if first is None:
+                first = got
+            else:
+                assert np.array_equal(got, first), (tiled, cols)
     finally:
         capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
+        capi.check(capi.lib().dq_set_tuning(b"spmm_cols_per_lane", 0))
 
 
 @pytest.mark.parametrize("M,K,R,offs", [
@@ -527,6 +545,9 @@ def test_spmm_dia_device(d, case, complex_):
     (3000, 2000, 1, [-2999, -700, 0, 511, 512, 1999]),  # one column: tiles of 256..1024 rows, halo edges
     (700, 900, 1500, [-5, 0, 5]),                      # wider than a tile: the flat kernel
     (2048, 2048, 3, [-900, 1200]),                     # no diagonal near the tile: no window
+    (4096, 4096, 16, list(range(-20, 21))),            # 41 diagonals: several unrolled groups and a ragged one
+    (1025, 1100, 128, [-1024, -3, 0, 9, 1099]),        # wide: 16 chunks of 8 columns per row
+    (333, 333, 12, [-5, 1, 4]),                        # 12 columns: chunks of 4
 ])
 def test_spmm_dia_device_tiling_edges(d, M, K, R, offs):
     import torch
@@ -541,17 +562,7 @@ def test_spmm_dia_device_tiling_edges(d, M, K, R, offs):
     tB = torch.as_tensor(B, device="cuda").contiguous()
     tC = torch.full((M, R), float("nan"), dtype=torch.complex128, device="cuda")
     off = torch.tensor(offs, dtype=torch.int64, device="cuda")
-    from diaq_amd import capi
-    try:
-        for tiled in (0, 1, 2, 4):
-            capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", tiled))
-            tC.fill_(float("nan"))
-            d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(), True,
-                              torch.cuda.current_stream().cuda_stream)
-            torch.cuda.synchronize()
-            assert rel(tC.cpu().numpy(), ref) < 1e-13, tiled
-    finally:
-        capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
+    spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, True, ref)
 
 
 def test_device_block_cache_reuse(d):
