@@ -200,6 +200,10 @@ bool dense_set_tuning(const std::string &key, int value);
 bool convert_set_tuning(const std::string &key, int value);
 // Gate-pass tuning key ("gates_tile_qubits"); false if it is another key.
 bool gates_set_tuning(const std::string &key, int value);
+// y (+)= (I_{2^before} (x) G (x) I_{2^after}) x on device vectors of G's precision (kron.hip); G square, checked by
+// the caller; x == y allowed when G is at most 16 x 16 with at most 64 diagonals.
+void kron_apply(const dq_devmat &G, int before, int after, const void *xr, const void *xi, void *yr, void *yi,
+                bool accumulate, hipStream_t s);
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);

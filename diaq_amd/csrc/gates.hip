@@ -6,7 +6,7 @@
 //
 // A tile is the set of amplitudes that agree on every qubit outside the pass's tile qubits T (|T| = L). T holds the
 // qubits of the stretch's non-diagonal gates, filled up with the lowest remaining qubits; the planner keeps qubits
-// 0..LOW_QUBITS-1 in T, so each tile is made of runs of at least 2^LOW_QUBITS contiguous amplitudes. A gate whose only
+// 0..low-1 in T (low = 5 by default), so each tile is made of runs of at least 2^low contiguous amplitudes. A gate whose only
 // diagonal is the main one joins any pass: it multiplies each amplitude by the entry that the amplitude's own index
 // selects, wherever its qubits lie.
 //
@@ -23,20 +23,22 @@
 
 namespace dq {
 
-constexpr int LOW_QUBITS = 6;   // qubits 0..5 always in a tile: runs of 64 contiguous amplitudes
 constexpr int GATES_MAX_K = 4;  // gates on up to four qubits (16 x 16)
 constexpr int PASS_MAX_GATES = 64, PASS_MAX_ROWPTR = 512, PASS_MAX_TERMS = 512;  // per pass, staged in LDS
 
 // A gate as the kernel sees it. Its terms are listed row by row: row r of the gate adds, in the order of the gate's
 // diagonal slots, value * x[c] for each term (c, value) in [rowptr[rp + r], rowptr[rp + r + 1]) (indexes local to the
-// pass). kind 0: general (its qubits are tile bits p .. p + k - 1); 1: main diagonal only, applied per amplitude; 2:
-// main diagonal only on qubits outside the tile, so one factor per tile.
+// pass). kind 0: a one- or two-qubit general gate on tile bits p .. p + k - 1; 1: main diagonal only, applied per
+// amplitude; 2: main diagonal only on qubits outside the tile, so one factor per tile. Non-diagonal gates on three or
+// four qubits get a pass of their own, run by the single-gate kernel (the fused kernel then needs no registers for
+// 8- and 16-row columns).
 struct GateOp {
     int k, q, p, kind, rp;
 };
 
 struct PassHdr {
     int n_gates, n_rowptr, n_terms, nc;
+    int single, single_q;      // single >= 0: this pass is gate `single` of the plan's own copies, on qubits from single_q
     int tpos[16];              // tile bit i -> global qubit (ascending)
     int cpos[48];              // tile index bit m -> global qubit, for the qubits outside the tile (ascending)
     int64_t gop, rowptr, term;  // first GateOp / rowptr entry / term of this pass in the plan's arrays
@@ -132,20 +134,26 @@ __global__ __launch_bounds__(256) void gates_pass_kernel(const PassHdr *__restri
         __syncthreads();
         for (int j = 0; j < ng; ++j) {
             const GateOp G = sg[j];
-            if (G.kind != 0) {
+            if (G.kind == 0) {
+                __syncthreads();
+                if (G.k == 1) tile_gate<T, L, 2>(G, srp, st, sr, si);
+                else tile_gate<T, L, 4>(G, srp, st, sr, si);
+                own = false;
+            } else {
                 // a run of diagonal gates: each amplitude goes through all of them in registers, gate by gate with
                 // the same rounding as one at a time (kind 2, qubits outside the tile: the index bits come from
                 // `base` alone, so c is the same for the whole tile)
                 if (!own) __syncthreads();
                 own = true;
                 int j2 = j;
-                while (j2 + 1 < ng && sg[j2 + 1].kind != 0) ++j2;
+                while (j2 + 1 < ng && (sg[j2 + 1].kind == 1 || sg[j2 + 1].kind == 2)) ++j2;
                 T xr[E], xi[E];
 #pragma unroll
                 for (int m = 0; m < E; ++m) {
                     xr[m] = sr[threadIdx.x + 256 * m];
                     xi[m] = si[threadIdx.x + 256 * m];
                 }
+#pragma unroll 1
                 for (int jj = j; jj <= j2; ++jj) {
                     const GateOp D = sg[jj];
                     const GateTerm<T> *row = st + srp[D.rp];  // one term per row: row c's term is row[c]
@@ -165,13 +173,6 @@ __global__ __launch_bounds__(256) void gates_pass_kernel(const PassHdr *__restri
                     si[threadIdx.x + 256 * m] = xi[m];
                 }
                 j = j2;
-            } else {
-                __syncthreads();
-                if (G.k == 1) tile_gate<T, L, 2>(G, srp, st, sr, si);
-                else if (G.k == 2) tile_gate<T, L, 4>(G, srp, st, sr, si);
-                else if (G.k == 3) tile_gate<T, L, 8>(G, srp, st, sr, si);
-                else tile_gate<T, L, 16>(G, srp, st, sr, si);
-                own = false;
             }
         }
         __syncthreads();
@@ -192,14 +193,21 @@ __global__ __launch_bounds__(256) void gates_pass_kernel(const PassHdr *__restri
 struct dq_gate_plan {
     int n_qubits = 0, precision = DQ_F64, L = 11;
     std::vector<dq::PassHdr> passes;
+    std::vector<std::unique_ptr<dq_devmat>> big;  // copies of the non-diagonal gates on three or four qubits
     dq::DevBuf d_passes, d_gops, d_rowptr, d_terms;
 };
 
 namespace dq {
 
 static int g_gates_tile_qubits = 11;  // dq_set_tuning "gates_tile_qubits": 10, 11 or 12
+static int g_gates_low_qubits = 5;    // dq_set_tuning "gates_low_qubits": 4, 5 or 6 (qubits always in a tile)
 
 bool gates_set_tuning(const std::string &key, int value) {
+    if (key == "gates_low_qubits") {
+        if (value < 4 || value > 6) throw ArgError("gates_low_qubits must be 4, 5 or 6");
+        g_gates_low_qubits = value;
+        return true;
+    }
     if (key != "gates_tile_qubits") return false;
     if (value < 10 || value > 12) throw ArgError("gates_tile_qubits must be 10, 11 or 12");
     g_gates_tile_qubits = value;
@@ -256,17 +264,20 @@ static HostGate host_gate(const dq_devmat &G, int k, int q) {
 }
 
 // Greedy cut of the sequence into passes: a non-diagonal gate joins the current pass while the qubits of the pass's
-// non-diagonal gates, together with qubits 0..LOW_QUBITS-1, number at most L, and while the pass's tables fit in LDS.
+// non-diagonal gates, together with qubits 0..low-1, number at most L, and while the pass's tables fit in LDS.
 template <typename T>
-static void plan_passes(dq_gate_plan &P, const std::vector<HostGate> &gates) {
+static void plan_passes(dq_gate_plan &P, const std::vector<HostGate> &gates,
+                        const std::vector<const dq_devmat *> &devgates) {
     const int n = P.n_qubits, L = P.L;
     std::vector<GateOp> ops;
     std::vector<int> rowptr;
     std::vector<GateTerm<T>> terms;
     auto close = [&](int g0, int g1, uint64_t S) {
+        if (g1 <= g0) return;
         uint64_t Tm = S;
         for (int b = 0; b < n && __builtin_popcountll(Tm) < L; ++b) Tm |= uint64_t(1) << b;
         PassHdr D{};
+        D.single = -1;
         int i = 0, c = 0;
         for (int b = 0; b < n; ++b) {
             if ((Tm >> b) & 1) D.tpos[i++] = b;
@@ -303,11 +314,24 @@ static void plan_passes(dq_gate_plan &P, const std::vector<HostGate> &gates) {
         D.n_terms = nt;
         P.passes.push_back(D);
     };
-    const uint64_t low = (uint64_t(1) << LOW_QUBITS) - 1;
+    const uint64_t low = (uint64_t(1) << g_gates_low_qubits) - 1;
     uint64_t S = 0;
     int g0 = 0, nrp = 0, nt = 0;
     for (int j = 0; j < static_cast<int>(gates.size()); ++j) {
         const HostGate &H = gates[j];
+        if (!H.diag && H.k > 2) {  // a pass of its own: the single-gate kernel, on a copy the plan keeps
+            if (devgates[j]->n() > 64) throw ArgError("gates: a gate has more than 64 diagonals");
+            close(g0, j, S);
+            PassHdr D{};
+            D.single = static_cast<int>(P.big.size());
+            D.single_q = H.q;
+            P.big.push_back(copy_devmat(*devgates[j], nullptr));
+            P.passes.push_back(D);
+            g0 = j + 1;
+            S = 0;
+            nrp = nt = 0;
+            continue;
+        }
         const uint64_t Q = H.diag ? 0 : ((uint64_t(1) << H.k) - 1) << H.q;
         const int hr = static_cast<int>(H.rowptr.size()), ht = static_cast<int>(H.col.size());
         if (ht > PASS_MAX_TERMS) throw ArgError("gates: a gate has more than 512 nonzero terms");
@@ -323,13 +347,15 @@ static void plan_passes(dq_gate_plan &P, const std::vector<HostGate> &gates) {
         nt += ht;
     }
     close(g0, static_cast<int>(gates.size()), S);
+    DQ_HIP(hipDeviceSynchronize());  // the copies of the large gates are complete
     P.d_passes = DevBuf(sizeof(PassHdr) * P.passes.size());
-    P.d_gops = DevBuf(sizeof(GateOp) * ops.size());
-    P.d_rowptr = DevBuf(sizeof(int) * rowptr.size());
+    P.d_gops = DevBuf(sizeof(GateOp) * std::max<size_t>(1, ops.size()));
+    P.d_rowptr = DevBuf(sizeof(int) * std::max<size_t>(1, rowptr.size()));
     P.d_terms = DevBuf(sizeof(GateTerm<T>) * std::max<size_t>(1, terms.size()));
     DQ_HIP(hipMemcpy(P.d_passes.p, P.passes.data(), P.d_passes.bytes, hipMemcpyHostToDevice));
-    DQ_HIP(hipMemcpy(P.d_gops.p, ops.data(), P.d_gops.bytes, hipMemcpyHostToDevice));
-    DQ_HIP(hipMemcpy(P.d_rowptr.p, rowptr.data(), P.d_rowptr.bytes, hipMemcpyHostToDevice));
+    if (!ops.empty()) DQ_HIP(hipMemcpy(P.d_gops.p, ops.data(), sizeof(GateOp) * ops.size(), hipMemcpyHostToDevice));
+    if (!rowptr.empty())
+        DQ_HIP(hipMemcpy(P.d_rowptr.p, rowptr.data(), sizeof(int) * rowptr.size(), hipMemcpyHostToDevice));
     if (!terms.empty())
         DQ_HIP(hipMemcpy(P.d_terms.p, terms.data(), sizeof(GateTerm<T>) * terms.size(), hipMemcpyHostToDevice));
 }
@@ -343,15 +369,25 @@ static void run_passes(const dq_gate_plan &P, void *re, void *im, hipStream_t s)
     const int *dr = P.d_rowptr.as<int>();
     const GateTerm<T> *dt = P.d_terms.as<GateTerm<T>>();
     for (size_t i = 0; i < P.passes.size(); ++i) {
-        if (P.L == 10)
-            hipLaunchKernelGGL((gates_pass_kernel<T, 10>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,
-                               static_cast<T *>(re), static_cast<T *>(im), ntiles);
-        else if (P.L == 12)
-            hipLaunchKernelGGL((gates_pass_kernel<T, 12>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,
-                               static_cast<T *>(re), static_cast<T *>(im), ntiles);
-        else
-            hipLaunchKernelGGL((gates_pass_kernel<T, 11>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,
-                               static_cast<T *>(re), static_cast<T *>(im), ntiles);
+        if (P.passes[i].single >= 0) {
+            const dq_devmat &G = *P.big[P.passes[i].single];
+            int k = 0;
+            while ((1u << k) < G.rows) ++k;
+            const int q = P.passes[i].single_q;
+            kron_apply(G, P.n_qubits - q - k, q, re, im, re, im, false, s);
+            continue;
+        }
+#define DQ_GATES_PASS(LL)                                                                                         \
+    hipLaunchKernelGGL((gates_pass_kernel<T, LL>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,                \
+                       static_cast<T *>(re), static_cast<T *>(im), ntiles)
+        if (P.L == 10) {
+            DQ_GATES_PASS(10);
+        } else if (P.L == 12) {
+            DQ_GATES_PASS(12);
+        } else {
+            DQ_GATES_PASS(11);
+        }
+#undef DQ_GATES_PASS
         DQ_LAUNCH_CHECK();
     }
 }
@@ -382,8 +418,9 @@ extern "C" int dq_gate_plan_create(int n_qubits, int n_gates, const dq_devmat *c
             if (first_qubit[j] + k > static_cast<uint32_t>(n_qubits)) throw ArgError("gates: gate beyond n_qubits");
             hg.push_back(host_gate(*G, k, static_cast<int>(first_qubit[j])));
         }
-        if (P->precision == DQ_F64) plan_passes<double>(*P, hg);
-        else plan_passes<float>(*P, hg);
+        std::vector<const dq_devmat *> dg(gates, gates + n_gates);
+        if (P->precision == DQ_F64) plan_passes<double>(*P, hg, dg);
+        else plan_passes<float>(*P, hg, dg);
         *out = P.release();
     });
 }

@@ -151,6 +151,12 @@ static int64_t kron_check(const dq_devmat *G, uint32_t before, uint32_t after) {
 
 static bool kron_in_place_ok(const dq_devmat &G) { return G.rows <= KRON_REG_MAX && G.n() <= KRON_MAX_SLOTS; }
 
+void kron_apply(const dq_devmat &G, int before, int after, const void *xr, const void *xi, void *yr, void *yi,
+                bool accumulate, hipStream_t s) {
+    if (G.precision == DQ_F64) kron_launch<double>(G, before, after, xr, xi, yr, yi, accumulate, s);
+    else kron_launch<float>(G, before, after, xr, xi, yr, yi, accumulate, s);
+}
+
 }  // namespace dq
 
 using namespace dq;

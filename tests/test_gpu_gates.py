@@ -142,3 +142,24 @@ def test_gate_plan_errors(d):
         d.GatePlan(12, [(G2, 0), (G32, 1)])
     with pytest.raises(RuntimeError, match="beyond"):
         d.GatePlan(12, [(G2, 12)])
+
+
+@pytest.mark.parametrize("low", [4, 6])
+def test_gate_plan_low_qubits_bitwise(d, low):
+    """Other low-qubit counts per tile (the default is 5): the same bits as gate by gate."""
+    import torch
+    from diaq_amd import capi
+    capi.check(capi.lib().dq_set_tuning(b"gates_low_qubits", low))
+    try:
+        rng = np.random.default_rng(30 + low)
+        n = 15
+        seq, _ = random_sequence(d, rng, n, 40)
+        P = d.GatePlan(n, seq)
+        ar, ai = state(torch, n, 31)
+        br, bi = ar.clone(), ai.clone()
+        P.run(ar.data_ptr(), ai.data_ptr())
+        sequential(d, seq, n, br, bi)
+        torch.cuda.synchronize()
+        assert torch.equal(ar, br) and torch.equal(ai, bi)
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"gates_low_qubits", 5))
