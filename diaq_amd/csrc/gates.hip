@@ -186,12 +186,200 @@ __global__ __launch_bounds__(256) void gates_pass_kernel(const PassHdr *__restri
     }
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Register layout: a lane keeps its E amplitudes of the tile (tile element e = threadIdx.x + 256 m) in registers for
+// the whole pass. Tile bits 0..5 are lane bits within a wave, 6..7 the wave, 8.. the register index m. A gate on lane
+// bits takes its partners by cross-lane shuffles, one on register bits from the lane's own registers, and only a
+// gate touching a wave bit exchanges through LDS. Each lane computes the rows of the amplitudes it owns, with the
+// same terms in the same order as the single-gate kernel, so the bits are the same.
+// ------------------------------------------------------------------------------------------------------------------
+// Row r of a gate over operands v[0..GM): the terms of the row in slot order.
+template <typename T, int GM>
+__device__ inline void gate_row(const int *rowptr, const GateTerm<T> *terms, int rp, int r, const T (&vr)[GM],
+                                const T (&vi)[GM], T &out_r, T &out_i) {
+    Acc<T> ar, ai;
+    const int t1 = rowptr[rp + r + 1];
+    for (int t = rowptr[rp + r]; t < t1; ++t) {
+        const GateTerm<T> w = terms[t];
+        T xr = vr[0], xi = vi[0];
+#pragma unroll
+        for (int c = 1; c < GM; ++c) {
+            if (w.c == c) {
+                xr = vr[c];
+                xi = vi[c];
+            }
+        }
+        cmac(ar, ai, w.re, w.im, xr, xi);
+    }
+    out_r = ar.get();
+    out_i = ai.get();
+}
+
+// A gate of 2^K rows on tile bits [p, p + K) inside the lane bits (p + K <= 6): partners by shuffles.
+template <typename T, int E, int K>
+__device__ inline void regs_gate_lanes(const GateOp &G, const int *rowptr, const GateTerm<T> *terms, T (&xr)[E],
+                                       T (&xi)[E]) {
+    constexpr int GM = 1 << K;
+    const int lane = threadIdx.x & 63;
+    const int r = (lane >> G.p) & (GM - 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        // partner d (d = c xor r, 1 <= d < GM) sits in lane lane ^ (d << p): one exchange per partner, the same
+        // masks for every lane
+        T wr[GM], wi[GM];
+        wr[0] = xr[m];
+        wi[0] = xi[m];
+#pragma unroll
+        for (int dd = 1; dd < GM; ++dd) {
+            wr[dd] = __shfl_xor(xr[m], dd << G.p);
+            wi[dd] = __shfl_xor(xi[m], dd << G.p);
+        }
+        T vr[GM], vi[GM];
+#pragma unroll
+        for (int c = 0; c < GM; ++c) {
+            vr[c] = wr[0];
+            vi[c] = wi[0];
+#pragma unroll
+            for (int dd = 1; dd < GM; ++dd) {
+                if ((c ^ r) == dd) {
+                    vr[c] = wr[dd];
+                    vi[c] = wi[dd];
+                }
+            }
+        }
+        gate_row<T, GM>(rowptr, terms, G.rp, r, vr, vi, xr[m], xi[m]);
+    }
+}
+
+// A gate of 2^K rows on register bits [RB, RB + K) of m: the partners are the lane's own registers.
+template <typename T, int E, int K, int RB>
+__device__ inline void regs_gate_regs(const GateOp &G, const int *rowptr, const GateTerm<T> *terms, T (&xr)[E],
+                                      T (&xi)[E]) {
+    constexpr int GM = 1 << K;
+    if constexpr (RB + K <= 31 && (E >> (RB + K)) >= 1) {
+#pragma unroll
+        for (int m0 = 0; m0 < E; ++m0) {
+            if (m0 & ((GM - 1) << RB)) continue;
+            T vr[GM], vi[GM], yr[GM], yi[GM];
+#pragma unroll
+            for (int c = 0; c < GM; ++c) {
+                vr[c] = xr[m0 | (c << RB)];
+                vi[c] = xi[m0 | (c << RB)];
+            }
+#pragma unroll
+            for (int r = 0; r < GM; ++r) gate_row<T, GM>(rowptr, terms, G.rp, r, vr, vi, yr[r], yi[r]);
+#pragma unroll
+            for (int c = 0; c < GM; ++c) {
+                xr[m0 | (c << RB)] = yr[c];
+                xi[m0 | (c << RB)] = yi[c];
+            }
+        }
+    }
+}
+
+template <typename T, int E, int K, int RB>
+__device__ inline void regs_gate_regs_dispatch(int rb, const GateOp &G, const int *rowptr, const GateTerm<T> *terms,
+                                               T (&xr)[E], T (&xi)[E]) {
+    if constexpr ((1 << (RB + K)) <= E) {
+        if (rb == RB) regs_gate_regs<T, E, K, RB>(G, rowptr, terms, xr, xi);
+        else regs_gate_regs_dispatch<T, E, K, RB + 1>(rb, G, rowptr, terms, xr, xi);
+    }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void gates_regs_kernel(const PassHdr *__restrict__ hdr,
+                                                         const GateOp *__restrict__ gops,
+                                                         const int *__restrict__ rowptrs,
+                                                         const GateTerm<T> *__restrict__ terms_g, T *re, T *im,
+                                                         int64_t ntiles) {
+    constexpr int E = (1 << L) / 256;  // amplitudes per lane
+    __shared__ T sr[1 << L], si[1 << L];  // only for gates that touch the wave bits 6 and 7
+    __shared__ GateOp sg[PASS_MAX_GATES];
+    __shared__ int srp[PASS_MAX_ROWPTR];
+    __shared__ GateTerm<T> st[PASS_MAX_TERMS];
+    const PassHdr &P = *hdr;
+    const int ng = P.n_gates;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) sg[i] = gops[P.gop + i];
+    for (int i = threadIdx.x; i < P.n_rowptr; i += blockDim.x) srp[i] = rowptrs[P.rowptr + i];
+    for (int i = threadIdx.x; i < P.n_terms; i += blockDim.x) st[i] = terms_g[P.term + i];
+    int64_t off[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        int64_t v = 0;
+#pragma unroll
+        for (int i = 0; i < L; ++i) v |= static_cast<int64_t>((e >> i) & 1) << P.tpos[i];
+        off[m] = v;
+    }
+    __syncthreads();
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int64_t base = 0;
+        for (int i = 0; i < P.nc; ++i) base |= ((t >> i) & 1) << P.cpos[i];
+        T xr[E], xi[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            xr[m] = re[base + off[m]];
+            xi[m] = im[base + off[m]];
+        }
+        for (int j = 0; j < ng; ++j) {
+            const GateOp G = sg[j];
+            if (G.kind != 0) {  // diagonal: per amplitude, from its own index
+                const GateTerm<T> *row = st + srp[G.rp];
+                const int mask = (1 << G.k) - 1;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const GateTerm<T> w = row[static_cast<int>(((base + off[m]) >> G.q) & mask)];
+                    Acc<T> ar, ai;
+                    cmac(ar, ai, w.re, w.im, xr[m], xi[m]);
+                    xr[m] = ar.get();
+                    xi[m] = ai.get();
+                }
+            } else if (G.p + G.k <= 6) {
+                if (G.k == 1) regs_gate_lanes<T, E, 1>(G, srp, st, xr, xi);
+                else regs_gate_lanes<T, E, 2>(G, srp, st, xr, xi);
+            } else if (G.p >= 8) {
+                if (G.k == 1) regs_gate_regs_dispatch<T, E, 1, 0>(G.p - 8, G, srp, st, xr, xi);
+                else regs_gate_regs_dispatch<T, E, 2, 0>(G.p - 8, G, srp, st, xr, xi);
+            } else {  // touches a wave bit: exchange through LDS
+                __syncthreads();  // earlier readers of the LDS copy are done
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    sr[threadIdx.x + 256 * m] = xr[m];
+                    si[threadIdx.x + 256 * m] = xi[m];
+                }
+                __syncthreads();
+                const int GM = 1 << G.k;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const int e = threadIdx.x + 256 * m;
+                    const int r = (e >> G.p) & (GM - 1);
+                    Acc<T> ar, ai;
+                    const int t1 = srp[G.rp + r + 1];
+                    for (int tt = srp[G.rp + r]; tt < t1; ++tt) {
+                        const GateTerm<T> w = st[tt];
+                        const int a = e ^ ((w.c ^ r) << G.p);
+                        cmac(ar, ai, w.re, w.im, sr[a], si[a]);
+                    }
+                    xr[m] = ar.get();
+                    xi[m] = ai.get();
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            re[base + off[m]] = xr[m];
+            im[base + off[m]] = xi[m];
+        }
+        __syncthreads();  // the next tile's LDS exchanges start after every lane is done with this one's
+    }
+}
+
 }  // namespace dq
 
 // A planned gate sequence: passes, gate ops, row pointers and terms (copies of the gates' values), built once and run
 // any number of times.
 struct dq_gate_plan {
-    int n_qubits = 0, precision = DQ_F64, L = 11;
+    int n_qubits = 0, precision = DQ_F64, L = 11, kernel = 0;
     std::vector<dq::PassHdr> passes;
     std::vector<std::unique_ptr<dq_devmat>> big;  // copies of the non-diagonal gates on three or four qubits
     dq::DevBuf d_passes, d_gops, d_rowptr, d_terms;
@@ -201,8 +389,14 @@ namespace dq {
 
 static int g_gates_tile_qubits = 11;  // dq_set_tuning "gates_tile_qubits": 10, 11 or 12
 static int g_gates_low_qubits = 5;    // dq_set_tuning "gates_low_qubits": 4, 5 or 6 (qubits always in a tile)
+static int g_gates_kernel = 0;        // dq_set_tuning "gates_kernel": 0 = LDS tiles, 1 = register layout
 
 bool gates_set_tuning(const std::string &key, int value) {
+    if (key == "gates_kernel") {
+        if (value != 0 && value != 1) throw ArgError("gates_kernel must be 0 or 1");
+        g_gates_kernel = value;
+        return true;
+    }
     if (key == "gates_low_qubits") {
         if (value < 4 || value > 6) throw ArgError("gates_low_qubits must be 4, 5 or 6");
         g_gates_low_qubits = value;
@@ -378,8 +572,12 @@ static void run_passes(const dq_gate_plan &P, void *re, void *im, hipStream_t s)
             continue;
         }
 #define DQ_GATES_PASS(LL)                                                                                         \
-    hipLaunchKernelGGL((gates_pass_kernel<T, LL>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,                \
-                       static_cast<T *>(re), static_cast<T *>(im), ntiles)
+    if (P.kernel == 1)                                                                                            \
+        hipLaunchKernelGGL((gates_regs_kernel<T, LL>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,            \
+                           static_cast<T *>(re), static_cast<T *>(im), ntiles);                                    \
+    else                                                                                                          \
+        hipLaunchKernelGGL((gates_pass_kernel<T, LL>), dim3(grid), dim3(256), 0, s, dp + i, dg, dr, dt,            \
+                           static_cast<T *>(re), static_cast<T *>(im), ntiles)
         if (P.L == 10) {
             DQ_GATES_PASS(10);
         } else if (P.L == 12) {
@@ -408,6 +606,7 @@ extern "C" int dq_gate_plan_create(int n_qubits, int n_gates, const dq_devmat *c
         auto P = std::make_unique<dq_gate_plan>();
         P->n_qubits = n_qubits;
         P->L = L;
+        P->kernel = g_gates_kernel;
         std::vector<HostGate> hg;
         for (int j = 0; j < n_gates; ++j) {
             const dq_devmat *G = gates[j];

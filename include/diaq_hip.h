@@ -167,8 +167,9 @@ int dq_release_cached_memory(void);
  * exponential goes through a dense library GEMM); "dense_convert" (0 = automatic, 1 = per-row kernels, 2 = tiled
  * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "dense_convert_tile" (0 =
  * 32 x 64 tiles, the default, 1 = 64 x 32, 2 = 64 x 64, 3 = 128 x 32) their tiled kernels' tile; "gates_tile_qubits"
- * (10, 11 = default, 12) the tile of later gate plans and "gates_low_qubits" (4, 5, 6 = default) the low qubits that
- * every tile of theirs keeps (runs of 2^low contiguous amplitudes); "alloc_cache_mb" (-1 = an
+ * (10, 11 = default, 12) the tile of later gate plans, "gates_low_qubits" (4, 5 = default, 6) the low qubits that
+ * every tile of theirs keeps (runs of 2^low contiguous amplitudes) and "gates_kernel" (0 = LDS tiles, the default;
+ * 1 = amplitudes in registers with cross-lane shuffles) their pass kernel; "alloc_cache_mb" (-1 = an
  * eighth of the device memory, 0 = no cache) bounds the cache of freed device blocks. No reference counterpart. */
 int dq_set_tuning(const char *key, int value);
 

@@ -49,12 +49,15 @@ def sequential(d, seq, n, vr, vi):
         d.kron_apply_device(G, n - q - k, q, vr.data_ptr(), vi.data_ptr(), vr.data_ptr(), vi.data_ptr(), False)
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("tile", [10, 11, 12])
 @pytest.mark.parametrize("n,n_gates,seed", [(12, 25, 1), (14, 40, 2), (16, 60, 3), (13, 1, 4), (15, 8, 5)])
-def test_gate_plan_bitwise_equals_single_gates(d, tile, n, n_gates, seed):
+def test_gate_plan_bitwise_equals_single_gates(d, kernel, tile, n, n_gates, seed):
+    """Both pass kernels (LDS tiles; amplitudes in registers with shuffles) at each tile size."""
     import torch
     from diaq_amd import capi
     capi.check(capi.lib().dq_set_tuning(b"gates_tile_qubits", tile))
+    capi.check(capi.lib().dq_set_tuning(b"gates_kernel", kernel))
     try:
         rng = np.random.default_rng(seed * 10 + tile)
         seq, _ = random_sequence(d, rng, n, n_gates)
@@ -68,6 +71,7 @@ def test_gate_plan_bitwise_equals_single_gates(d, tile, n, n_gates, seed):
         assert torch.equal(ar, br) and torch.equal(ai, bi)
     finally:
         capi.check(capi.lib().dq_set_tuning(b"gates_tile_qubits", 11))
+        capi.check(capi.lib().dq_set_tuning(b"gates_kernel", 0))
 
 
 def test_gate_plan_vs_oracle_and_repetitions(d):
@@ -88,8 +92,18 @@ def test_gate_plan_vs_oracle_and_repetitions(d):
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
 
 
-def test_gate_plan_fp32_bitwise(d):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gate_plan_fp32_bitwise(d, kernel):
     import torch
+    from diaq_amd import capi
+    capi.check(capi.lib().dq_set_tuning(b"gates_kernel", kernel))
+    try:
+        fp32_case(d, torch)
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"gates_kernel", 0))
+
+
+def fp32_case(d, torch):
     n = 14
     rng = np.random.default_rng(9)
     seq, _ = random_sequence(d, rng, n, 30, precision="fp32")
@@ -102,11 +116,21 @@ def test_gate_plan_fp32_bitwise(d):
     assert torch.equal(ar, br) and torch.equal(ai, bi)
 
 
-def test_trotter_step_plan_full_size(d):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_trotter_step_plan_full_size(d, kernel):
     """Ten first-order Trotter steps of TFIM-24 (27 + 28 gates per step in the 28-qubit bench; 23 + 24 here) as one
     plan run with repetitions: the same bits as the gates one by one, norm kept to 1e-12, and the ZZ layer fused
-    into the first pass (passes << gates)."""
+    into the first pass (passes << gates); both pass kernels."""
     import torch
+    from diaq_amd import capi
+    capi.check(capi.lib().dq_set_tuning(b"gates_kernel", kernel))
+    try:
+        trotter_case(d, torch)
+    finally:
+        capi.check(capi.lib().dq_set_tuning(b"gates_kernel", 0))
+
+
+def trotter_case(d, torch):
     n = 24
     dt, J, h = 0.05, 1.0, 0.7
     zz = np.exp(-1j * dt * J * np.array([1.0, -1.0, -1.0, 1.0]))
