@@ -559,7 +559,7 @@ def trotter_gates(d, n, dt, J=1.0, h=0.7):
 def trotter_section(args, torch, d, capi, Hm, n=28):
     """First-order Trotter steps of the 28-qubit TFIM (J = 1, h = 0.7, dt = 0.05) on the device: every step applies
     27 ZZ rotations and 28 X rotations in place to a 2^28-amplitude state (split re/im, FP64; 4.3 GB), once gate by
-    gate (dq_kron_apply_device) and once as a fused plan (dq_gate_plan: 5 passes per step). Algorithmic bytes per
+    gate (dq_kron_apply_device) and once as a fused plan (dq_gate_plan: 4 passes per step). Algorithmic bytes per
     pass over the state: 32 * 2^n (read and written once). CPU baseline: the reference's host loop nest
     (oracle_kron_apply_tasks, y += G x) on a 2^24-amplitude sample of one ZZ and one X gate."""
     N = 1 << n
