@@ -126,6 +126,12 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     except Exception as e:  # the headline stays valid without the FP32 figure
         res["fp32"] = dict(error=str(e))
 
+    if getattr(args, "vendor", True):
+        try:
+            res["vendor_csr"] = vendor_csr_spmv(args, torch, re, im, lens, offs, xr, xi, yr, yi, N, stream)
+        except Exception as e:  # context only: the headline stays valid without it
+            res["vendor_csr"] = dict(error=str(e))
+            log(f"[bench] vendor CSR SpMV failed: {e}")
     if not args.no_cpu:
         try:
             res["cpu"] = cpu_baseline(args, torch, re, im, lens, offs, xr, xi, yr, yi, N)
@@ -136,6 +142,53 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     L.dq_devmat_destroy(out)
     torch.cuda.synchronize()
     return res
+
+
+def vendor_csr_spmv(args, torch, re, im, lens, offs, xr, xi, yr, yi, N, stream):
+    """The same matrix in CSR (int32 indices, complex128 values) multiplied by the vendor library: torch's sparse CSR
+    matmul, i.e. hipSPARSE / rocSPARSE SpMV/SpMM on ROCm. Context for the DIA kernel, not a baseline of the
+    reference (whose GPU path is DIA only). GB/s is quoted on the DIA algorithmic bytes of the same product, so the
+    two rates compare times directly; the CSR kernel itself must move 20 B per nonzero."""
+    dev = "cuda"
+    parts_r, parts_c = [], []
+    for o, ln in zip(offs, lens):
+        rlo = -o if o < 0 else 0
+        r = torch.arange(rlo, rlo + ln, dtype=torch.int64, device=dev)
+        parts_r.append(r)
+        parts_c.append(r + o)
+    rows = torch.cat(parts_r)
+    cols = torch.cat(parts_c)
+    del parts_r, parts_c
+    order = torch.argsort(rows * N + cols)
+    cols_s = cols[order].to(torch.int32)
+    vals_s = torch.complex(re[order], im[order])
+    crow = torch.zeros(N + 1, dtype=torch.int32, device=dev)
+    crow[1:] = torch.cumsum(torch.bincount(rows, minlength=N), 0).to(torch.int32)
+    del rows, cols, order
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # torch marks sparse CSR as beta
+        A = torch.sparse_csr_tensor(crow, cols_s, vals_s, size=(N, N))
+    x = torch.complex(xr, xi).unsqueeze(1)
+    y = A @ x
+    torch.cuda.synchronize()
+    ref = torch.complex(yr, yi)
+    err = float((y[:, 0] - ref).abs().max() / ref.abs().max())
+    reps = max(1, min(args.steps, 10))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        y = A @ x
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    S = int(sum(lens))
+    gbs = (16 * S + 32 * N) / (ms * 1e-3) / 1e9
+    log(f"[bench] vendor CSR SpMV (torch sparse CSR, int32 indices): {ms:.3f} ms = {gbs:.0f} GB/s on the DIA bytes; "
+        f"max rel diff vs DIA {err:.1e}")
+    del A, cols_s, vals_s, crow, x, y
+    torch.cuda.empty_cache()
+    return dict(ms=ms, gbs_on_dia_bytes=gbs, csr_bytes=20 * S + 4 * (N + 1) + 32 * N, max_rel_diff_vs_dia=err)
 
 
 def cpu_baseline(args, torch, re, im, lens, offs, xr, xi, yr, yi, N):
@@ -799,6 +852,7 @@ def main():
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    args.vendor = args.profile is None  # the vendor CSR comparison runs in the full bench only
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
                  "run with --gpus 1")
@@ -892,6 +946,7 @@ def main():
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
                          else None),
         "spmv_fp32": sp.get("fp32"),
+        "spmv_vendor_csr": sp.get("vendor_csr"),
         "cpu_detail": cpu or None,
         "spmv_structured": extra.get("spmv_structured"),
         "evolution": evo,
