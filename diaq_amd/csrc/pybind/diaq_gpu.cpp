@@ -89,6 +89,44 @@ struct HostIdentity : HostDia {  // src/specialMatrices.hpp:9-24
     }
 };
 
+// permutationMatrix(s, row_order): P[i, row_order[i]] = 1, stored as the ascending nonzero diagonals of that dense
+// matrix, as the reference builds it (src/specialMatrices.hpp:26-51); entries with row_order[i] >= s are absent.
+struct HostPermutation : HostDia {
+    HostPermutation(unsigned s, const std::vector<unsigned> &row_order) {
+        if (row_order.size() != s) throw std::invalid_argument("permutationMatrix: row_order needs s entries");
+        std::vector<int> ds;
+        for (unsigned i = 0; i < s; ++i)
+            if (row_order[i] < s) ds.push_back(static_cast<int>(row_order[i]) - static_cast<int>(i));
+        std::sort(ds.begin(), ds.end());
+        ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+        matrix_shape = {s, s};
+        shape = {s, s};
+        dIndices = ds;
+        init_storage();
+        for (unsigned i = 0; i < s; ++i) {
+            if (row_order[i] >= s) continue;
+            const int d = static_cast<int>(row_order[i]) - static_cast<int>(i);
+            const size_t k = std::lower_bound(ds.begin(), ds.end(), d) - ds.begin();
+            const unsigned pos = d >= 0 ? i : row_order[i];  // position p holds A[p, p+d] (d >= 0) or A[p-d, p]
+            re[diagOffsets[k] + pos] = 1.0;
+        }
+    }
+};
+
+// denseMatrix: the reference's dense container (src/denseMatrix.hpp:8-170), kept on the host; convertToDiaQ applies
+// the dense -> ascending nonzero diagonals rule (src/dense_to_diagonals.hpp:20-58).
+struct HostDense {
+    std::array<unsigned, 2> matrix_shape{0, 0};
+    std::vector<unsigned> shape;
+    std::vector<double> re, im;
+    HostDense(unsigned s, std::vector<double> r, std::vector<double> i)
+        : matrix_shape{s, s}, shape{s, s}, re(std::move(r)), im(std::move(i)) {
+        const size_t n = static_cast<size_t>(s) * s;
+        if (re.size() != n || im.size() != n)
+            throw std::invalid_argument("denseMatrix: the value arrays need s * s entries");
+    }
+};
+
 struct HostVector {  // src/vector.hpp:8-299 (the parts the GPU path uses)
     unsigned numRows = 0;
     std::vector<double> re, im;
@@ -453,6 +491,77 @@ PYBIND11_MODULE(diaq_gpu, m) {
         });
 
     py::class_<HostIdentity, HostDia, std::shared_ptr<HostIdentity>>(m, "identityMatrix").def(py::init<unsigned>());
+    py::class_<HostPermutation, HostDia, std::shared_ptr<HostPermutation>>(m, "permutationMatrix")
+        .def(py::init<unsigned, const std::vector<unsigned> &>());
+
+    auto dense_values = [](py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+        return std::vector<double>(a.data(), a.data() + a.size());
+    };
+    auto dense_to_dia = [](const HostDense &D, std::array<unsigned, 2> ms) {
+        const size_t n = D.re.size();
+        if (static_cast<size_t>(ms[0]) * ms[1] != n)
+            throw std::invalid_argument("convertToDiaQ: the new shape must hold the same number of entries");
+        std::vector<cplx> v(n);
+        for (size_t i = 0; i < n; ++i) v[i] = cplx(D.re[i], D.im[i]);
+        return diagonals_from_dense(ms, D.shape, v.data());
+    };
+    py::class_<HostDense, std::shared_ptr<HostDense>>(m, "denseMatrix")
+        .def(py::init([dense_values](unsigned s, py::array_t<double, py::array::c_style | py::array::forcecast> r) {
+            return std::make_shared<HostDense>(s, dense_values(r), std::vector<double>(r.size(), 0.0));
+        }))
+        .def(py::init([dense_values](unsigned s, py::array_t<double, py::array::c_style | py::array::forcecast> r,
+                                     py::array_t<double, py::array::c_style | py::array::forcecast> i) {
+            return std::make_shared<HostDense>(s, dense_values(r), dense_values(i));
+        }))
+        .def_readwrite("matrix_shape", &HostDense::matrix_shape)
+        .def_readwrite("shape", &HostDense::shape)
+        .def("getItem", [](const HostDense &D, const std::vector<unsigned> &pos) {  // getFlatIndex, denseMatrix.hpp
+            if (pos.size() != D.shape.size()) throw std::invalid_argument("getItem: one index per tensor axis");
+            unsigned long long flat = 0, prod = 1;
+            const unsigned long long size = static_cast<unsigned long long>(D.matrix_shape[0]) * D.matrix_shape[1];
+            for (size_t i = 0; i < D.shape.size(); ++i) {
+                prod *= D.shape[i];
+                flat += (size / prod) * pos[i];
+            }
+            if (flat >= D.re.size()) throw std::out_of_range("getItem: index outside the matrix");
+            return cplx(D.re[flat], D.im[flat]);
+        })
+        .def("set", [](HostDense &D, unsigned i, unsigned j, cplx v) {
+            const size_t f = static_cast<size_t>(D.matrix_shape[1]) * i + j;
+            D.re.at(f) = v.real();
+            D.im.at(f) = v.imag();
+        })
+        .def("get", [](const HostDense &D, unsigned i, unsigned j) {
+            const size_t f = static_cast<size_t>(D.matrix_shape[1]) * i + j;
+            return cplx(D.re.at(f), D.im.at(f));
+        })
+        .def("print", [](const HostDense &D) {
+            py::print("matrix_shape (" + std::to_string(D.matrix_shape[0]) + "," + std::to_string(D.matrix_shape[1]) +
+                      ")");
+            for (unsigned i = 0; i < D.matrix_shape[0]; ++i) {
+                std::string line;
+                for (unsigned j = 0; j < D.matrix_shape[1]; ++j) {
+                    const size_t f = static_cast<size_t>(D.matrix_shape[1]) * i + j;
+                    line += (j ? "," : "") + std::string("(") + std::to_string(D.re[f]) + "," +
+                            std::to_string(D.im[f]) + ")";
+                }
+                py::print(line);
+            }
+        })
+        .def("isPartOf", [](const HostDense &D, int dIndex, int a, int b, int x, int y) {
+            const long long size = static_cast<long long>(D.matrix_shape[0]) * D.matrix_shape[1];
+            const int p = dIndex >= size ? x + y - a - b : x - y + b - a;
+            return p == 0;
+        })
+        .def("sparsity", [](const HostDense &D) {
+            size_t z = 0;
+            for (size_t i = 0; i < D.re.size(); ++i) z += (D.re[i] == 0.0 && D.im[i] == 0.0);
+            return D.re.empty() ? 0.0 : 100.0 * static_cast<double>(z) / static_cast<double>(D.re.size());
+        })
+        .def("convertToDiaQ", [dense_to_dia](const HostDense &D, std::vector<unsigned> s) {
+            return dense_to_dia(D, pair_of(s));
+        }, py::arg())
+        .def("convertToDiaQ", [dense_to_dia](const HostDense &D) { return dense_to_dia(D, D.matrix_shape); });
 
     py::class_<HostVector, std::shared_ptr<HostVector>>(m, "vector")
         .def(py::init<unsigned>())

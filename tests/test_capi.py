@@ -202,3 +202,33 @@ def test_gate_plan_argument_errors_need_no_gpu():
     n = ctypes.c_int()
     assert L.dq_gate_plan_passes(None, ctypes.byref(n)) == capi.DQ_ERR_ARG
     assert L.dq_gate_plan_run(None, None, None, 1, 0) == capi.DQ_ERR_ARG
+
+
+def test_permutation_and_dense_matrix_classes():
+    """permutationMatrix (src/specialMatrices.hpp:26-51) and the denseMatrix container (src/denseMatrix.hpp)."""
+    import diaq_amd  # noqa: F401
+    import diaq_gpu as d
+    order = [3, 0, 4, 1, 2, 7, 6, 5]
+    P = d.permutationMatrix(8, order)
+    dense = np.zeros((8, 8))
+    dense[np.arange(8), order] = 1
+    np.testing.assert_array_equal(d.to_numpy(P), dense)
+    assert list(P.dIndices) == O.from_dense(dense).offsets  # ascending nonzero diagonals, as the dense route gives
+    rng = np.random.default_rng(4)
+    R = rng.standard_normal((6, 6)) * (rng.random((6, 6)) < 0.4)
+    I = rng.standard_normal((6, 6)) * (rng.random((6, 6)) < 0.4)
+    D = d.denseMatrix(6, R.ravel(), I.ravel())
+    assert list(D.matrix_shape) == [6, 6] and list(D.shape) == [6, 6]
+    assert D.get(2, 3) == complex(R[2, 3], I[2, 3]) and D.getItem([2, 3]) == complex(R[2, 3], I[2, 3])
+    D.set(1, 1, 2 - 3j)
+    R[1, 1], I[1, 1] = 2, -3
+    A = D.convertToDiaQ()
+    np.testing.assert_array_equal(d.to_numpy(A), R + 1j * I)
+    assert list(A.dIndices) == O.from_dense(R + 1j * I).offsets
+    assert abs(D.sparsity() - 100.0 * np.mean((R == 0) & (I == 0))) < 1e-12
+    B = D.convertToDiaQ([4, 9])  # the same entries read as a 4 x 9 matrix
+    np.testing.assert_array_equal(d.to_numpy(B), (R + 1j * I).reshape(4, 9))
+    with pytest.raises(ValueError):
+        D.convertToDiaQ([5, 5])
+    with pytest.raises(ValueError):
+        d.denseMatrix(3, np.ones(8))
