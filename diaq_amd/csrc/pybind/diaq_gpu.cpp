@@ -488,7 +488,67 @@ PYBIND11_MODULE(diaq_gpu, m) {
         .def("print_meta", [](const HostDia &A) {
             std::cout << "matrix_shape (" << A.matrix_shape[0] << "," << A.matrix_shape[1] << ") diagonals "
                       << A.dIndices.size() << " totalLength " << A.totalLength << std::endl;
-        });
+        })
+        .def("print", [](const HostDia &A) {  // src/diaQMatrix.hpp:749-790
+            std::string t;
+            for (size_t i = 0; i < A.shape.size(); ++i) t += (i ? "," : "") + std::to_string(A.shape[i]);
+            py::print("matrix_shape (" + std::to_string(A.matrix_shape[0]) + "," + std::to_string(A.matrix_shape[1]) +
+                      ")");
+            py::print("tensor_shape (" + t + ")");
+            py::print("#Diagonals = " + std::to_string(A.dIndices.size()));
+            for (size_t k = 0; k < A.dIndices.size(); ++k) {
+                std::string line = "(dIndex=" + std::to_string(A.dIndices[k]) + ") (dLength=" +
+                                   std::to_string(A.len(k)) + ") [";
+                for (unsigned j = 0; j < A.len(k); ++j)
+                    line += (j ? ",(" : "(") + std::to_string(A.re[A.diagOffsets[k] + j]) + "," +
+                            std::to_string(A.im[A.diagOffsets[k] + j]) + ")";
+                py::print(line + "]");
+            }
+        })
+        .def("sparsity", [](const HostDia &A) {  // src/diaQMatrix.hpp:1361-1375: 1 - stored nonzeros / size
+            size_t nnz = 0;
+            for (size_t t = 0; t < A.re.size(); ++t) nnz += (A.re[t] != 0.0 || A.im[t] != 0.0);
+            const double size = static_cast<double>(A.matrix_shape[0]) * A.matrix_shape[1];
+            return 1.0 - static_cast<double>(nnz) / size;
+        })
+        .def("convertToDenseMatrix", [](const HostDia &A) {  // src/conversions.cpp:11-48
+            const size_t n = static_cast<size_t>(A.matrix_shape[0]) * A.matrix_shape[1];
+            auto D = std::make_shared<HostDense>(0u, std::vector<double>{}, std::vector<double>{});
+            D->matrix_shape = A.matrix_shape;
+            D->shape = A.shape;
+            D->re.assign(n, 0.0);
+            D->im.assign(n, 0.0);
+            for (size_t k = 0; k < A.dIndices.size(); ++k) {
+                const int d = A.dIndices[k];
+                for (unsigned j = 0; j < A.len(k); ++j) {
+                    const size_t r = d >= 0 ? j : j - d, c = d >= 0 ? j + d : j;
+                    D->re[r * A.matrix_shape[1] + c] = A.re[A.diagOffsets[k] + j];
+                    D->im[r * A.matrix_shape[1] + c] = A.im[A.diagOffsets[k] + j];
+                }
+            }
+            return D;
+        })
+        .def("addMatrixScaled", [](HostDia &A, std::shared_ptr<HostDia> B, double rs, double is) {
+            // A = A + (rs + i is) B over the ascending union of their diagonals (src/diaQMatrix.hpp:1061-1142)
+            if (A.matrix_shape != B->matrix_shape) throw std::runtime_error("addMatrixScaled: matrix shapes do not match");
+            std::vector<int> u(A.dIndices);
+            u.insert(u.end(), B->dIndices.begin(), B->dIndices.end());
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            HostDia C(A.matrix_shape, u, A.shape);
+            for (size_t k = 0; k < u.size(); ++k) {
+                const int sa = A.slot(u[k]), sb = B->slot(u[k]);
+                for (unsigned j = 0; j < C.len(k); ++j) {
+                    const double ar = sa >= 0 ? A.re[A.diagOffsets[sa] + j] : 0.0;
+                    const double ai = sa >= 0 ? A.im[A.diagOffsets[sa] + j] : 0.0;
+                    const double br = sb >= 0 ? B->re[B->diagOffsets[sb] + j] : 0.0;
+                    const double bi = sb >= 0 ? B->im[B->diagOffsets[sb] + j] : 0.0;
+                    C.re[C.diagOffsets[k] + j] = ar + (rs * br - is * bi);
+                    C.im[C.diagOffsets[k] + j] = ai + (rs * bi + is * br);
+                }
+            }
+            A = C;
+        }, py::arg("B"), py::arg("real_scale_factor"), py::arg("imag_scale_factor"));
 
     py::class_<HostIdentity, HostDia, std::shared_ptr<HostIdentity>>(m, "identityMatrix").def(py::init<unsigned>());
     py::class_<HostPermutation, HostDia, std::shared_ptr<HostPermutation>>(m, "permutationMatrix")

@@ -232,3 +232,19 @@ def test_permutation_and_dense_matrix_classes():
         D.convertToDiaQ([5, 5])
     with pytest.raises(ValueError):
         d.denseMatrix(3, np.ones(8))
+
+
+def test_host_dia_sparsity_dense_and_scaled_add():
+    """diaQMatrix.sparsity / convertToDenseMatrix / addMatrixScaled (src/diaQMatrix.hpp:1061-1142, 1361-1375)."""
+    import diaq_amd  # noqa: F401
+    import diaq_gpu as d
+    rng = np.random.default_rng(5)
+    A = (rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))) * (rng.random((7, 7)) < 0.3)
+    B = (rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))) * (rng.random((7, 7)) < 0.3)
+    dA, dB = d.from_numpy(A), d.from_numpy(B)
+    assert abs(dA.sparsity() - (1 - np.count_nonzero(A) / 49)) < 1e-15
+    Dn = dA.convertToDenseMatrix()
+    assert list(Dn.matrix_shape) == [7, 7] and Dn.get(3, 4) == A[3, 4]
+    dA.addMatrixScaled(dB, 0.5, -2.0)
+    np.testing.assert_allclose(d.to_numpy(dA), A + (0.5 - 2j) * B, rtol=0, atol=1e-15)
+    assert list(dA.dIndices) == sorted(set(O.from_dense(A).offsets) | set(O.from_dense(B).offsets))
