@@ -633,6 +633,18 @@ PYBIND11_MODULE(diaq_gpu, m) {
             return out;
         })
         .def("copy", [](const HostVector &v) { return std::make_shared<HostVector>(v); })
+        .def("isEqualTo", [](const HostVector &v, std::shared_ptr<HostVector> w, double tol) {  // src/vector.hpp:119-131
+            if (v.numRows != w->numRows) return false;
+            for (unsigned i = 0; i < v.numRows; ++i)
+                if (std::abs(v.re[i] - w->re[i]) > tol || std::abs(v.im[i] - w->im[i]) > tol) return false;
+            return true;
+        }, py::arg("v"), py::arg("tol") = 1e-6)
+        .def("print", [](const HostVector &v) {
+            std::string line;
+            for (unsigned i = 0; i < v.numRows; ++i)
+                line += (i ? ",(" : "(") + std::to_string(v.re[i]) + "," + std::to_string(v.im[i]) + ")";
+            py::print(line);
+        })
         .def("norm", [](const HostVector &v) {
             double s = 0;
             for (unsigned i = 0; i < v.numRows; ++i) s += v.re[i] * v.re[i] + v.im[i] * v.im[i];
