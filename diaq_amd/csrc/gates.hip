@@ -6,9 +6,9 @@
 //
 // A tile is the set of amplitudes that agree on every qubit outside the pass's tile qubits T (|T| = L). T holds the
 // qubits of the stretch's non-diagonal gates, filled up with the lowest remaining qubits; the planner keeps qubits
-// 0..low-1 in T (low = 5 by default), so each tile is made of runs of at least 2^low contiguous amplitudes. A gate whose only
-// diagonal is the main one joins any pass: it multiplies each amplitude by the entry that the amplitude's own index
-// selects, wherever its qubits lie.
+// 0..low-1 in T (low = 5 by default), so each tile is made of runs of at least 2^low contiguous amplitudes. A gate
+// whose only diagonal is the main one joins any pass: it multiplies each amplitude by the entry that the amplitude's
+// own index selects, wherever its qubits lie.
 //
 // Each gate's arithmetic on each amplitude is the single-gate kernel's (each row adds its terms in the order of the
 // gate's diagonal slots, with the rounding sequence of dia_rows.hpp), and the order of the gates is kept, so the
