@@ -45,6 +45,9 @@ struct AccT {
 // LDSW: the workgroup first stages the input rows [tile - T/2, tile + 3T/2) (T rows per tile, all NS states, zero
 // outside the matrix) in LDS, and the diagonals with |offset| <= T/2 gather from there: the +-2^k offsets of a qubit
 // Hamiltonian then share one copy of the 8-state vectors per tile.
+// zmask (NULL = off): bit b of word [job][diagonal][b / 32] is clear when every value of the diagonal in rows
+// [b * 256/NS, (b + 1) * 256/NS) is zero. Such a block adds nothing, so its value loads and x gathers are skipped;
+// a qubit Hamiltonian's +-2^k flip diagonals are zero on alternate runs of 2^k rows, half of what they store.
 template <typename TV, int NS, int RPT, bool LDSW>
 __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
                                                      int64_t total, const TV *__restrict__ hre,
@@ -53,7 +56,8 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
                                                      float *__restrict__ p32r, float *__restrict__ p32i,
                                                      const float *__restrict__ bin_r, const float *__restrict__ bin_i,
                                                      float *__restrict__ bout_r, float *__restrict__ bout_i,
-                                                     int64_t ntiles, int xcd) {
+                                                     int64_t ntiles, int xcd, const uint32_t *__restrict__ zmask,
+                                                     int64_t zwords) {
     const int j = blockIdx.y;
     const JobParam P = params[j];
     if (g >= P.steps) return;
@@ -94,12 +98,20 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
         const DiagMeta md = meta[d];
         const int64_t lo = md.rlo, hi_ = md.rhi;
         if (hi_ <= tile || lo >= tile + T) continue;  // uniform
+        // pass q of the tile covers block t*RPT + q of RPP rows; RPT divides 32, so the tile's bits share one word
+        uint32_t live = (1u << RPT) - 1;
+        if (zmask) {
+            const uint32_t w = zmask[(static_cast<int64_t>(j) * n + d) * zwords + ((t * RPT) >> 5)];
+            live = (w >> ((t * RPT) & 31)) & ((1u << RPT) - 1);
+            if (!live) continue;  // uniform: every value of the diagonal in this tile is zero
+        }
         const TV *ar = hre + vb + md.rowbase;
         const TV *ai = him + vb + md.rowbase;
-        TV va_r[RPT], va_i[RPT], vx_r[RPT], vx_i[RPT];
+        TV va_r[RPT] = {}, va_i[RPT] = {}, vx_r[RPT] = {}, vx_i[RPT] = {};
         if (LDSW && md.off >= -H && md.off <= H) {  // uniform: gather from the staged window
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
+                if (!((live >> q) & 1u)) continue;  // uniform
                 const int64_t r = tile + rl + RPP * q;
                 const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);  // stays inside the tile
                 va_r[q] = ar[rc];
@@ -111,6 +123,7 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
         } else {
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
+                if (!((live >> q) & 1u)) continue;  // uniform
                 const int64_t r = tile + rl + RPP * q;
                 const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);
                 va_r[q] = ar[rc];
@@ -123,7 +136,7 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
             const int64_t r = tile + rl + RPP * q;
-            if (r >= lo && r < hi_) cmac(hr[q], hi[q], va_r[q], va_i[q], vx_r[q], vx_i[q]);
+            if (((live >> q) & 1u) && r >= lo && r < hi_) cmac(hr[q], hi[q], va_r[q], va_i[q], vx_r[q], vx_i[q]);
         }
     }
 #pragma unroll
@@ -146,6 +159,35 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
             bout_r[o] = static_cast<float>(nr);
             bout_i[o] = static_cast<float>(ni);
         }
+    }
+}
+
+// The zero-block map of horner_kernel: one workgroup per (word, diagonal, job), 32 blocks of RPP rows each. A block
+// is live when a value of the diagonal inside [rlo, rhi) and inside the block is nonzero (a NaN counts as nonzero).
+template <typename TV, int RPP>
+__global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
+                                                          int64_t total, const TV *__restrict__ hre,
+                                                          const TV *__restrict__ him, int64_t zwords,
+                                                          uint32_t *__restrict__ zmask) {
+    __shared__ int live[32];
+    if (threadIdx.x < 32) live[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t w = blockIdx.x;
+    const int d = blockIdx.y, j = blockIdx.z;
+    const DiagMeta md = meta[d];
+    const TV *ar = hre + static_cast<int64_t>(j) * total + md.rowbase;
+    const TV *ai = him + static_cast<int64_t>(j) * total + md.rowbase;
+    const int64_t r0 = w * 32 * RPP;
+    for (int e = threadIdx.x; e < 32 * RPP; e += 256) {
+        const int64_t r = r0 + e;
+        if (r >= md.rlo && r < md.rhi && r < N && !(ar[r] == TV(0) && ai[r] == TV(0))) live[e / RPP] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; ++b)
+            if (live[b]) m |= 1u << b;
+        zmask[(static_cast<int64_t>(j) * n + d) * zwords + w] = m;
     }
 }
 
@@ -294,6 +336,8 @@ struct EvoClass {
     DevBuf d_sq;              // [job][NS] squared norms
     DevBuf d_partial;         // [job][block][NS] partial squared norms
     DevBuf d_probe;           // [job][NS][2]
+    DevBuf zmask;             // [job][diagonal][zwords] live-block bits of horner_kernel
+    int64_t zwords = 0;
     std::vector<JobParam> params;
     int64_t G = 0;            // launches: max steps
     int rpt = 4;              // tuned rows per lane
@@ -329,6 +373,7 @@ namespace dq {
 static int g_evo_rpt = 0;   // 0 = the class's, else 1, 2, 4 or 8
 static int g_evo_xcd = -1;  // -1 = the class's, else 0 or 1
 static int g_evo_lds = -1;  // -1 = the class's, else 0 or 1
+static int g_evo_skip = 1;  // skip the all-zero blocks of a diagonal ("evo_skip_zero_blocks")
 
 bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_rows_per_lane") {
@@ -345,6 +390,11 @@ bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_lds") {
         if (value < -1 || value > 1) throw ArgError("evo_lds must be -1, 0 or 1");
         g_evo_lds = value;
+        return true;
+    }
+    if (key == "evo_skip_zero_blocks") {
+        if (value != 0 && value != 1) throw ArgError("evo_skip_zero_blocks must be 0 or 1");
+        g_evo_skip = value;
         return true;
     }
     return false;
@@ -364,11 +414,12 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     float *bout_i = (g & 1) ? c.b1i.as<float>() : c.b0i.as<float>();
     const float *bin_r = (g & 1) ? c.b0r.as<float>() : c.b1r.as<float>();
     const float *bin_i = (g & 1) ? c.b0i.as<float>() : c.b1i.as<float>();
+    const uint32_t *zmask = g_evo_skip ? c.zmask.as<uint32_t>() : nullptr;
 #define DQ_HORNER_L(R, L)                                                                                        \
     hipLaunchKernelGGL((horner_kernel<TV, NS, R, L>), grid, dim3(256), 0, s, N, (int)c.offs.size(),               \
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
                        g, c.p64r.as<double>(), c.p64i.as<double>(), c.p32r.as<float>(), c.p32i.as<float>(), bin_r,   \
-                       bin_i, bout_r, bout_i, ntiles, xcd)
+                       bin_i, bout_r, bout_i, ntiles, xcd, zmask, c.zwords)
 #define DQ_HORNER(R)                         \
     if (lds) { DQ_HORNER_L(R, true); }       \
     else { DQ_HORNER_L(R, false); }
@@ -525,6 +576,24 @@ extern "C" int dq_evo_create(int n_jobs, dq_devmat *const *H, int n_states, int 
             DQ_HIP(hipMemcpy(static_cast<char *>(c.hre.p) + slot * bytes, T->re.p, bytes, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(static_cast<char *>(c.him.p) + slot * bytes, T->im.p, bytes, hipMemcpyDeviceToDevice));
         }
+        // the zero-block maps, from the values just copied
+        for (auto &c : E->classes) {
+            const int nd = static_cast<int>(c.offs.size());
+            const int64_t rpp = 256 / n_states;
+            c.zwords = ((static_cast<int64_t>(c.N) + rpp - 1) / rpp + 31) / 32;
+            c.zmask = DevBuf(sizeof(uint32_t) * std::max<int64_t>(1, c.zwords * nd * c.jobs_n()));
+            if (nd == 0 || c.jobs_n() == 0) continue;
+            const dim3 grid(static_cast<unsigned>(c.zwords), static_cast<unsigned>(nd), static_cast<unsigned>(c.jobs_n()));
+            dispatch(value_precision, n_states, [&](auto tv, auto ns) {
+                using TV = decltype(tv);
+                constexpr int RPP = 256 / decltype(ns)::value;
+                hipLaunchKernelGGL((zero_blocks_kernel<TV, RPP>), grid, dim3(256), 0, nullptr,
+                                   static_cast<int64_t>(c.N), nd, c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(),
+                                   c.him.as<TV>(), c.zwords, c.zmask.as<uint32_t>());
+            });
+            DQ_LAUNCH_CHECK();
+        }
+        DQ_HIP(hipDeviceSynchronize());
         *out = E.release();
     });
 }

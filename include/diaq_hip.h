@@ -160,7 +160,8 @@ int dq_release_cached_memory(void);
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
  * product's values) and "product_rows_per_lane" (4, 8 = default, 16: row chunks of 256 x that) set the SpGEMM
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
- * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule;
+ * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule,
+ * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
  * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
  * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)
  * pick the spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix

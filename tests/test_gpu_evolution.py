@@ -129,3 +129,52 @@ def test_every_horner_schedule_gives_the_same_bits(mods):
                                               for r, x, w in chosen)
     for a, b in zip(run(), base):
         np.testing.assert_array_equal(a, b)
+
+
+def test_zero_block_skip_matches_full_reads(mods):
+    """Blocks of a diagonal whose values are all zero (a TFIM's flip diagonals are zero on alternate runs of 2^k rows)
+    add nothing: skipping them agrees with reading them to FP32 rounding, stays schedule-independent bit for bit, and
+    a Hamiltonian with a whole zero diagonal (h = 0 on one site) evolves as the dense expm says."""
+    from diaq_amd import capi
+    d, ev = mods
+    L = capi.lib()
+    jobs = small_batch(d, seed=4, n_states=8)
+    rng = np.random.default_rng(5)
+    r, c, offs, vals = Hm.tfim(10, 1.1, 0.6)
+    vals = [np.array(v, dtype=complex) for v in vals]
+    for o in (1 << 7, -(1 << 7)):  # no field on qubit 7: both of its flip diagonals zero everywhere
+        vals[offs.index(o)][:] = 0.0
+    D = d.to_device(d.diaQMatrix.from_diagonals([r, c], offs, vals))
+    st = rng.standard_normal((8, r)) + 1j * rng.standard_normal((8, r))
+    st /= np.linalg.norm(st, axis=1, keepdims=True)
+    jobs.append(dict(n=10, t=0.7, D=D, dense=O.to_dense(O.Dia(r, c, offs, vals)), states=st))
+    sched = [ev.plain_rule(d.estimate_1norm(j["D"]), j["t"], 1e-8) for j in jobs]
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=8, value_precision="fp32")
+    B.set_schedule([j["t"] for j in jobs], [m for m, s in sched], [s for m, s in sched])
+
+    def run():
+        for q, j in enumerate(jobs):
+            B.set_states(q, j["states"])
+        B.run(use_graph=False, renormalize=True)
+        return [B.get_states(q, 1 << j["n"]) for q, j in enumerate(jobs)]
+
+    skip = run()
+    try:
+        capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 0))
+        full = run()
+        capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 1))
+        for rpt in (1, 8):
+            capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+            for a, b in zip(run(), skip):
+                np.testing.assert_array_equal(a, b)
+    finally:
+        capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
+    for a, b in zip(skip, full):
+        assert np.max(np.abs(a - b)) < 1e-6
+    for j, got in zip(jobs, skip):
+        U = scipy.linalg.expm(-1j * j["t"] * j["dense"])
+        for s_, g in zip(j["states"], got):
+            assert np.linalg.norm(g - U @ s_) < 1e-6
+    with pytest.raises(RuntimeError, match="evo_skip_zero_blocks"):
+        capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 2))
