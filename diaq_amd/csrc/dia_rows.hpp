@@ -96,6 +96,40 @@ __device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__res
     __syncthreads();
 }
 
+// Zero-block map of a DIA layout: bit b of word [job][diagonal][b / 32] is set when some value of the diagonal in
+// rows [b * BR, (b + 1) * BR) is nonzero (a NaN counts as nonzero). One workgroup per (word, diagonal, job); the
+// values of job j start at j * total. A block whose bit is clear adds nothing to any row sum, so the row-tile loops
+// can skip its value loads and x gathers: a qubit Hamiltonian's +-2^k flip diagonals are zero on alternate runs of
+// 2^k rows (three runs in four for a flip-flop term), half or more of what they store.
+template <typename T, int BR>
+__global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
+                                                          int64_t total, const T *__restrict__ are,
+                                                          const T *__restrict__ aim, int64_t zwords,
+                                                          uint32_t *__restrict__ zmask) {
+    __shared__ int live[32];
+    if (threadIdx.x < 32) live[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t w = blockIdx.x;
+    const int d = blockIdx.y, j = blockIdx.z;
+    const DiagMeta md = meta[d];
+    const T *pr = are + static_cast<int64_t>(j) * total + md.rowbase;
+    const T *pi = aim + static_cast<int64_t>(j) * total + md.rowbase;
+    const int64_t r0 = w * 32 * BR;
+    for (int e = threadIdx.x; e < 32 * BR; e += 256) {
+        const int64_t r = r0 + e;
+        if (r >= md.rlo && r < md.rhi && r < rows && !(pr[r] == T(0) && pi[r] == T(0))) live[e / BR] = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; ++b)
+            if (live[b]) m |= 1u << b;
+        zmask[(static_cast<int64_t>(j) * n + d) * zwords + w] = m;
+    }
+}
+
+inline int64_t zero_block_words(int64_t rows, int block_rows) { return ((rows + block_rows - 1) / block_rows + 31) / 32; }
+
 // PIPE = true: each diagonal's 4*RPT loads are issued together (full tiles unconditionally, boundary tiles with
 // clamped addresses), the schedule that keeps the most bytes in flight. Every schedule adds exactly the products of
 // the rows on the diagonal, in slot order, so a row's sum has the same bits whatever the schedule (a compensated
@@ -105,18 +139,30 @@ __device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__res
 //
 // With HALO > 0 the caller has staged the x window in (sr, si) with stage_x_window; diagonals with |offset| <= HALO
 // read x from there (row r of the tile at LDS index r - tile + HALO + offset), the others from memory.
-template <typename T, int RPT, bool NT = false, bool PIPE = true, int HALO = 0>
+//
+// ZM = true: zm is the layout's zero-block map at 256-row blocks (zero_blocks_kernel<T, 256>, zwords words per
+// diagonal; tiles start at a multiple of 256 * RPT rows) and the rows of a block whose values are all zero are
+// skipped -- the same blocks under every RPT, so every schedule still adds the same products.
+template <typename T, int RPT, bool NT = false, bool PIPE = true, int HALO = 0, bool ZM = false>
 __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, const DiagMeta *__restrict__ meta,
                                      const T *__restrict__ are, const T *__restrict__ aim, const T *__restrict__ xr,
                                      const T *__restrict__ xi, int64_t col_lo, int64_t col_hi, Acc<T> (&ar)[RPT],
                                      Acc<T> (&ai)[RPT], const T *__restrict__ sr = nullptr,
-                                     const T *__restrict__ si = nullptr) {
+                                     const T *__restrict__ si = nullptr, const uint32_t *__restrict__ zm = nullptr,
+                                     int64_t zwords = 0) {
     constexpr int TILE = 256 * RPT;
+    static_assert(!ZM || (RPT <= 16 && 32 % RPT == 0), "a tile's block bits must sit in one map word");
     for (int k = 0; k < n; ++k) {
         const DiagMeta m = meta[k];
         const int64_t lo = max(m.rlo, col_lo - m.off);
         const int64_t hi = min(m.rhi, col_hi - m.off);
         if (hi <= tile || lo >= tile_end) continue;  // uniform: the diagonal misses this tile
+        uint32_t live = (1u << RPT) - 1;             // bit j: rows tile + 256 j .. + 255 hold a nonzero value
+        if constexpr (ZM) {
+            const int64_t b0 = tile / 256;
+            live = (zm[static_cast<int64_t>(k) * zwords + (b0 >> 5)] >> (b0 & 31)) & ((1u << RPT) - 1);
+            if (!live) continue;  // uniform: the diagonal is all zero in this tile
+        }
         const T *pr = are + m.rowbase;
         const T *pi = aim + m.rowbase;
         if (HALO > 0 && m.off >= -HALO && m.off <= HALO) {  // uniform: x from the LDS window
@@ -125,6 +171,7 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
             if (lo <= tile && hi >= tile + TILE) {
 #pragma unroll
                 for (int j = 0; j < RPT; ++j) {
+                    if (ZM && !((live >> j) & 1u)) continue;  // uniform
                     const int64_t r = tile + threadIdx.x + 256 * j;
                     va_r[j] = ld_val<T, NT>(pr + r);
                     va_i[j] = ld_val<T, NT>(pi + r);
@@ -132,10 +179,12 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
                     vx_i[j] = si[threadIdx.x + 256 * j + base];
                 }
 #pragma unroll
-                for (int j = 0; j < RPT; ++j) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+                for (int j = 0; j < RPT; ++j)
+                    if (!ZM || ((live >> j) & 1u)) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
             } else {  // boundary tile: clamped rows stay inside the tile, so their LDS index stays in the window
 #pragma unroll
                 for (int j = 0; j < RPT; ++j) {
+                    if (ZM && !((live >> j) & 1u)) continue;  // uniform
                     const int64_t r = tile + threadIdx.x + 256 * j;
                     const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
                     va_r[j] = ld_val<T, NT>(pr + rc);
@@ -147,7 +196,8 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
 #pragma unroll
                 for (int j = 0; j < RPT; ++j) {  // rows off the diagonal add nothing (not even a zero)
                     const int64_t r = tile + threadIdx.x + 256 * j;
-                    if (r >= lo && r < hi) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+                    if ((!ZM || ((live >> j) & 1u)) && r >= lo && r < hi)
+                        cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
                 }
             }
             continue;
@@ -158,7 +208,7 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
                 const int64_t r = tile + threadIdx.x + 256 * j;
-                if (r >= lo && r < hi) {
+                if ((!ZM || ((live >> j) & 1u)) && r >= lo && r < hi) {
                     const T a_r = ld_val<T, NT>(pr + r), a_i = ld_val<T, NT>(pi + r), x_r = qr[r], x_i = qi[r];
                     cmac(ar[j], ai[j], a_r, a_i, x_r, x_i);
                 }
@@ -170,6 +220,7 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
         if (lo <= tile && hi >= tile + TILE) {  // uniform: every row of the tile is on the diagonal
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
+                if (ZM && !((live >> j) & 1u)) continue;  // uniform
                 const int64_t r = tile + threadIdx.x + 256 * j;
                 va_r[j] = ld_val<T, NT>(pr + r);
                 va_i[j] = ld_val<T, NT>(pi + r);
@@ -177,10 +228,12 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
                 vx_i[j] = qi[r];
             }
 #pragma unroll
-            for (int j = 0; j < RPT; ++j) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+            for (int j = 0; j < RPT; ++j)
+                if (!ZM || ((live >> j) & 1u)) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
         } else {  // boundary tile: clamp the loads to a valid row; rows off the diagonal add nothing (not even a zero)
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
+                if (ZM && !((live >> j) & 1u)) continue;  // uniform
                 const int64_t r = tile + threadIdx.x + 256 * j;
                 const int64_t rc = r < lo ? lo : (r >= hi ? hi - 1 : r);
                 va_r[j] = ld_val<T, NT>(pr + rc);
@@ -191,7 +244,8 @@ __device__ inline void dia_rows_tile(int64_t tile, int64_t tile_end, int n, cons
 #pragma unroll
             for (int j = 0; j < RPT; ++j) {
                 const int64_t r = tile + threadIdx.x + 256 * j;
-                if (r >= lo && r < hi) cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
+                if ((!ZM || ((live >> j) & 1u)) && r >= lo && r < hi)
+                    cmac(ar[j], ai[j], va_r[j], va_i[j], vx_r[j], vx_i[j]);
             }
         }
     }

@@ -141,6 +141,9 @@ struct dq_devmat {
     int tuned_rpt = 0, tuned_nt = -1, tuned_pipe = -1, tuned_lds = -1;
     // Row-tile order chosen by dq_spmv_tune (-1 = default round-robin, 0 = round-robin, 1 = XCD-contiguous).
     int tuned_xcd = -1;
+    // Fused-evolution term schedule, timed on the first evolution of this matrix (0 = not yet): rows per lane (2, 4,
+    // 8) and LDS x window (0, 1). A speed choice only: every schedule gives the same bits.
+    mutable int term_rpt = 0, term_lds = 0;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {
@@ -184,7 +187,8 @@ std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
 bool product_set_tuning(const std::string &key, int value);
 // spmm_dia tuning key ("spmm_tiled"); false if it is another key.
 bool spmm_set_tuning(const std::string &key, int value);
-// Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd", "evo_lds"); false if the key is not one of them.
+// Batched-evolution tuning keys ("evo_rows_per_lane", "evo_xcd", "evo_lds", "evo_skip_zero_blocks"); false if the
+// key is not one of them.
 bool evo_set_tuning(const std::string &key, int value);
 
 // C = (A B) / divisor with the structural dry-run (product.hip). allow_dense: a near-dense product may take the dense
@@ -200,6 +204,9 @@ bool dense_set_tuning(const std::string &key, int value);
 bool convert_set_tuning(const std::string &key, int value);
 // Gate-pass tuning key ("gates_tile_qubits"); false if it is another key.
 bool gates_set_tuning(const std::string &key, int value);
+// Fused-evolution tuning keys ("hamsim_skip_zero_blocks", "hamsim_rows_per_lane", "hamsim_lds"); false if the key is
+// not one of them.
+bool hamsim_set_tuning(const std::string &key, int value);
 // y (+)= (I_{2^before} (x) G (x) I_{2^after}) x on device vectors of G's precision (kron.hip); G square, checked by
 // the caller; x == y allowed when G is at most 16 x 16 with at most 64 diagonals.
 void kron_apply(const dq_devmat &G, int before, int after, const void *xr, const void *xi, void *yr, void *yi,

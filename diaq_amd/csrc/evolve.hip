@@ -162,35 +162,6 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
     }
 }
 
-// The zero-block map of horner_kernel: one workgroup per (word, diagonal, job), 32 blocks of RPP rows each. A block
-// is live when a value of the diagonal inside [rlo, rhi) and inside the block is nonzero (a NaN counts as nonzero).
-template <typename TV, int RPP>
-__global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
-                                                          int64_t total, const TV *__restrict__ hre,
-                                                          const TV *__restrict__ him, int64_t zwords,
-                                                          uint32_t *__restrict__ zmask) {
-    __shared__ int live[32];
-    if (threadIdx.x < 32) live[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t w = blockIdx.x;
-    const int d = blockIdx.y, j = blockIdx.z;
-    const DiagMeta md = meta[d];
-    const TV *ar = hre + static_cast<int64_t>(j) * total + md.rowbase;
-    const TV *ai = him + static_cast<int64_t>(j) * total + md.rowbase;
-    const int64_t r0 = w * 32 * RPP;
-    for (int e = threadIdx.x; e < 32 * RPP; e += 256) {
-        const int64_t r = r0 + e;
-        if (r >= md.rlo && r < md.rhi && r < N && !(ar[r] == TV(0) && ai[r] == TV(0))) live[e / RPP] = 1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t m = 0;
-        for (int b = 0; b < 32; ++b)
-            if (live[b]) m |= 1u << b;
-        zmask[(static_cast<int64_t>(j) * n + d) * zwords + w] = m;
-    }
-}
-
 // Forward Taylor terms of the first sub-step for the a-posteriori estimate: t_k = (Y/k) t_{k-1}, t_0 = psi (FP32);
 // the squared norms of t_m and t_{m+1} of every state are accumulated into norms[(j*NS + state)*2 + (k - m)].
 template <typename TV, int NS, int RPT>
@@ -580,7 +551,7 @@ extern "C" int dq_evo_create(int n_jobs, dq_devmat *const *H, int n_states, int 
         for (auto &c : E->classes) {
             const int nd = static_cast<int>(c.offs.size());
             const int64_t rpp = 256 / n_states;
-            c.zwords = ((static_cast<int64_t>(c.N) + rpp - 1) / rpp + 31) / 32;
+            c.zwords = zero_block_words(c.N, static_cast<int>(rpp));
             c.zmask = DevBuf(sizeof(uint32_t) * std::max<int64_t>(1, c.zwords * nd * c.jobs_n()));
             if (nd == 0 || c.jobs_n() == 0) continue;
             const dim3 grid(static_cast<unsigned>(c.zwords), static_cast<unsigned>(nd), static_cast<unsigned>(c.jobs_n()));

@@ -10,22 +10,52 @@
 // (Im h * s, -Re h * s) with s = dt/k, and adds term_k to the step's output; the first term reads psi directly and
 // writes out = psi + term_1, so the copies disappear. Per term: H read once, term_{k-1} read, term_k written and the
 // output read and written once (b*S + 4*b*N bytes). The same operation order as the reference per element:
-// row sums in slot order, then the scaling, then the add. The term kernel stages the x window of each row tile in LDS
-// when the matrix's SpMV schedule does (tuned, or some off-diagonals within half a tile): the +-2^k offsets of a qubit
-// Hamiltonian then share one copy of the term vector per tile.
+// row sums in slot order, then the scaling, then the add. The term kernel may stage the x window of each row tile in
+// LDS: the +-2^k offsets of a qubit Hamiltonian then share one copy of the term vector per tile. Its rows per lane
+// and LDS window are timed on the first evolution of a matrix and kept in the handle.
+//
+// Each call first maps the 256-row blocks of every diagonal whose values are all zero (one read of H), and every term
+// skips them: such a block adds nothing to a row sum (the products it would add are zeros, exact in FP64 for finite
+// inputs), and a qubit Hamiltonian's flip diagonals are zero on alternate runs of 2^k rows (three runs in four for a
+// flip-flop term). H cannot change during the call, so the map stays valid for all of its terms.
 #include <algorithm>
 
 #include "dia_rows.hpp"
 
 namespace dq {
 
-template <typename T, int RPT, bool LDS>
+static int g_hamsim_skip = 1;  // "hamsim_skip_zero_blocks"
+static int g_hamsim_rpt = 0;   // "hamsim_rows_per_lane": 0 = the matrix's timed choice, else 2, 4 or 8
+static int g_hamsim_lds = -1;  // "hamsim_lds": -1 = the matrix's timed choice, else 0 or 1
+
+bool hamsim_set_tuning(const std::string &key, int value) {
+    if (key == "hamsim_skip_zero_blocks") {
+        if (value != 0 && value != 1) throw ArgError("hamsim_skip_zero_blocks must be 0 or 1");
+        g_hamsim_skip = value;
+        return true;
+    }
+    if (key == "hamsim_rows_per_lane") {
+        if (value != 0 && value != 2 && value != 4 && value != 8)
+            throw ArgError("hamsim_rows_per_lane must be 0, 2, 4 or 8");
+        g_hamsim_rpt = value;
+        return true;
+    }
+    if (key == "hamsim_lds") {
+        if (value < -1 || value > 1) throw ArgError("hamsim_lds must be -1, 0 or 1");
+        g_hamsim_lds = value;
+        return true;
+    }
+    return false;
+}
+
+template <typename T, int RPT, bool LDS, bool ZM>
 __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
                                                           const T *__restrict__ are, const T *__restrict__ aim,
                                                           const T *__restrict__ tr, const T *__restrict__ ti,
                                                           T *__restrict__ nr, T *__restrict__ ni,
                                                           T *__restrict__ outr, T *__restrict__ outi, double scale,
-                                                          int first, int64_t ntiles, int xcd) {
+                                                          int first, int64_t ntiles, int xcd,
+                                                          const uint32_t *__restrict__ zm, int64_t zwords) {
     constexpr int TILE = 256 * RPT;
     constexpr int HALO = LDS ? TILE / 2 : 0;
     constexpr int W = LDS ? TILE + 2 * HALO : 1;
@@ -36,7 +66,8 @@ __global__ __launch_bounds__(256) void taylor_term_kernel(int64_t rows, int n, c
     const int64_t tile_end = tile + TILE < rows ? tile + TILE : rows;
     if constexpr (LDS) stage_x_window<T, RPT, HALO>(tile, rows, tr, ti, sr, si);
     Acc<T> hr[RPT], hi[RPT];
-    dia_rows_tile<T, RPT, true, true, HALO>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi, sr, si);
+    dia_rows_tile<T, RPT, true, true, HALO, ZM>(tile, tile_end, n, meta, are, aim, tr, ti, 0, rows, hr, hi, sr, si,
+                                                zm, zwords);
     const T s = static_cast<T>(scale);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
@@ -67,35 +98,87 @@ __global__ void copy2_kernel(int64_t n, const T *__restrict__ ar, const T *__res
 }
 
 // One step out = sum_{k=0}^{m} (-i dt H)^k psi / k! (src/spHamSim_gpu_all_fused.cu:235-277). t0/t1: scratch terms.
+// One term: term_k = (-i scale) H in, out (+)= term_k, on tiles of 256 * RPT rows.
+template <typename T, int RPT>
+static void launch_term(const dq_devmat &H, bool lds, const T *inr, const T *ini, T *nr, T *ni, T *outr, T *outi,
+                        double scale, int first, const uint32_t *zm, int64_t zwords, hipStream_t s) {
+    const int64_t N = H.rows;
+    const int64_t ntiles = (N + 256 * RPT - 1) / (256 * RPT);
+    const int xcd = xcd_tiles_for(H);
+    const dim3 grid(static_cast<unsigned>(grid_for_tiles(ntiles, xcd)));
+    const DiagMeta *meta = H.d_meta.as<DiagMeta>();
+    const T *are = static_cast<const T *>(H.re.p), *aim = static_cast<const T *>(H.im.p);
+#define DQ_TERM(L, Z)                                                                                             \
+    hipLaunchKernelGGL((taylor_term_kernel<T, RPT, L, Z>), grid, dim3(256), 0, s, N, H.n(), meta, are, aim, inr, ini, \
+                       nr, ni, outr, outi, scale, first, ntiles, xcd, zm, zwords)
+    if (lds && zm) DQ_TERM(true, true);
+    else if (lds) DQ_TERM(true, false);
+    else if (zm) DQ_TERM(false, true);
+    else DQ_TERM(false, false);
+#undef DQ_TERM
+    DQ_LAUNCH_CHECK();
+}
+
+template <typename T>
+using TermFn = void (*)(const dq_devmat &, bool, const T *, const T *, T *, T *, T *, T *, double, int,
+                        const uint32_t *, int64_t, hipStream_t);
+
+template <typename T>
+static TermFn<T> term_fn(int rpt) {
+    return rpt == 2 ? launch_term<T, 2> : rpt == 4 ? launch_term<T, 4> : launch_term<T, 8>;
+}
+
+// The term kernel's schedule for H: rows per lane {2, 4, 8} x LDS window {off, on where some off-diagonal lies
+// within half a tile}, each timed on two first terms (psi in, scratch out) the first time H is evolved, and kept in
+// the handle. Every schedule adds the same products in the same order, so the choice changes the speed only.
+template <typename T>
+static void term_schedule(const dq_devmat &H, const T *pr, const T *pi, T *outr, T *outi, T *tr, T *ti,
+                          const uint32_t *zm, int64_t zwords, hipStream_t s, int &rpt, bool &lds) {
+    if (H.term_rpt == 0 && (g_hamsim_rpt == 0 || g_hamsim_lds < 0)) {
+        hipEvent_t e0, e1;
+        DQ_HIP(hipEventCreate(&e0));
+        DQ_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        int brpt = 8, blds = 0;
+        for (int r : {2, 4, 8}) {
+            for (int l = 0; l < 2; ++l) {
+                if (l && !lds_window_pays(H, r)) continue;
+                const TermFn<T> f = term_fn<T>(r);
+                f(H, l == 1, pr, pi, tr, ti, outr, outi, 0.1, 1, zm, zwords, s);  // warm
+                DQ_HIP(hipEventRecord(e0, s));
+                for (int q = 0; q < 2; ++q) f(H, l == 1, pr, pi, tr, ti, outr, outi, 0.1, 1, zm, zwords, s);
+                DQ_HIP(hipEventRecord(e1, s));
+                DQ_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; brpt = r; blds = l; }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        H.term_rpt = brpt;
+        H.term_lds = blds;
+    }
+    rpt = g_hamsim_rpt ? g_hamsim_rpt : H.term_rpt;
+    lds = g_hamsim_lds >= 0 ? g_hamsim_lds == 1 : H.term_lds == 1;
+}
+
+// One step out = sum_{k=0}^{m} (-i dt H)^k psi / k! (src/spHamSim_gpu_all_fused.cu:235-277). t0/t1: scratch terms.
 template <typename T>
 static void taylor_step(const dq_devmat &H, const T *pr, const T *pi, T *outr, T *outi, T *t0r, T *t0i, T *t1r,
-                        T *t1i, double dt, int m, hipStream_t s) {
+                        T *t1i, double dt, int m, TermFn<T> term, bool lds, const uint32_t *zm, int64_t zwords,
+                        hipStream_t s) {
     const int64_t N = H.rows;
     if (m <= 0 || H.n() == 0) {  // out = psi (+ zero terms)
         hipLaunchKernelGGL(copy2_kernel<T>, dim3(blocks_for(N, 256, 8192)), dim3(256), 0, s, N, pr, pi, outr, outi);
         DQ_LAUNCH_CHECK();
         return;
     }
-    constexpr int RPT = 8;
-    const int64_t ntiles = (N + 256 * RPT - 1) / (256 * RPT);
-    const int xcd = xcd_tiles_for(H);
-    const unsigned grid = static_cast<unsigned>(grid_for_tiles(ntiles, xcd));
-    const bool lds = lds_window_for(H, RPT);
     const T *inr = pr, *ini = pi;
     T *bufr[2] = {t0r, t1r}, *bufi[2] = {t0i, t1i};
     for (int k = 1; k <= m; ++k) {
         T *nr = bufr[k & 1], *ni = bufi[k & 1];
-        if (lds)
-            hipLaunchKernelGGL((taylor_term_kernel<T, RPT, true>), dim3(grid), dim3(256), 0, s, N, H.n(),
-                               H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
-                               static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
-                               dt / static_cast<double>(k), k == 1 ? 1 : 0, ntiles, xcd);
-        else
-            hipLaunchKernelGGL((taylor_term_kernel<T, RPT, false>), dim3(grid), dim3(256), 0, s, N, H.n(),
-                               H.d_meta.as<DiagMeta>(), static_cast<const T *>(H.re.p),
-                               static_cast<const T *>(H.im.p), inr, ini, nr, ni, outr, outi,
-                               dt / static_cast<double>(k), k == 1 ? 1 : 0, ntiles, xcd);
-        DQ_LAUNCH_CHECK();
+        term(H, lds, inr, ini, nr, ni, outr, outi, dt / static_cast<double>(k), k == 1 ? 1 : 0, zm, zwords, s);
         inr = nr;
         ini = ni;
     }
@@ -109,9 +192,25 @@ static void evolve(const dq_devmat &H, T *pr, T *pi, double t, uint64_t r, int m
     const double dt = t / static_cast<double>(r);
     DevBuf br(N * sizeof(T)), bi(N * sizeof(T)), t0r(N * sizeof(T)), t0i(N * sizeof(T)), t1r(N * sizeof(T)),
         t1i(N * sizeof(T));
+    // the zero-block map of H (256-row blocks), read by every term of this call
+    const int64_t zwords = zero_block_words(N, 256);
+    DevBuf zmap;
+    if (g_hamsim_skip && H.n() > 0 && m > 0) {
+        zmap = DevBuf(sizeof(uint32_t) * zwords * H.n());
+        hipLaunchKernelGGL((zero_blocks_kernel<T, 256>), dim3(static_cast<unsigned>(zwords), H.n(), 1), dim3(256), 0,
+                           s, N, H.n(), H.d_meta.as<DiagMeta>(), int64_t(0), static_cast<const T *>(H.re.p),
+                           static_cast<const T *>(H.im.p), zwords, zmap.as<uint32_t>());
+        DQ_LAUNCH_CHECK();
+    }
+    const uint32_t *zm = zmap.p ? zmap.as<uint32_t>() : nullptr;
     T *ar = pr, *ai = pi, *xr = br.as<T>(), *xi = bi.as<T>();
+    int rpt = 8;
+    bool lds = false;
+    if (m > 0 && H.n() > 0) term_schedule<T>(H, pr, pi, xr, xi, t0r.as<T>(), t0i.as<T>(), zm, zwords, s, rpt, lds);
+    const TermFn<T> term = term_fn<T>(rpt);
     for (uint64_t step = 0; step < r; ++step) {
-        taylor_step<T>(H, ar, ai, xr, xi, t0r.as<T>(), t0i.as<T>(), t1r.as<T>(), t1i.as<T>(), dt, m, s);
+        taylor_step<T>(H, ar, ai, xr, xi, t0r.as<T>(), t0i.as<T>(), t1r.as<T>(), t1i.as<T>(), dt, m, term, lds, zm,
+                       zwords, s);
         std::swap(ar, xr);
         std::swap(ai, xi);
     }

@@ -224,7 +224,8 @@ extern "C" int dq_set_tuning(const char *key, int value) {
             if (value < -1) throw ArgError("alloc_cache_mb must be -1 (an eighth of the device memory) or >= 0");
             set_alloc_cache_budget(value);
         } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value) &&
-                   !dense_set_tuning(k, value) && !convert_set_tuning(k, value) && !gates_set_tuning(k, value)) {
+                   !dense_set_tuning(k, value) && !convert_set_tuning(k, value) && !gates_set_tuning(k, value) &&
+                   !hamsim_set_tuning(k, value)) {
             throw ArgError("unknown tuning key: " + k);
         }
     });

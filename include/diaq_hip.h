@@ -162,6 +162,8 @@ int dq_release_cached_memory(void);
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule,
  * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
+ * "hamsim_skip_zero_blocks" (1 = default, 0) does the same for the fused evolution's Taylor terms, whose schedule
+ * "hamsim_rows_per_lane" (0 = timed per matrix, or 2, 4, 8) and "hamsim_lds" (-1 = timed, 0, 1) override;
  * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
  * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)
  * pick the spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix

@@ -442,21 +442,68 @@ def test_hamsim_vs_oracle(d):
     psi = Hm.product_state(10, 2)
     H = dia_of(d, rows, cols, offs, vals)
     try:
-        for lds, xcd in ((0, 0), (1, 0), (0, 1), (1, 1), (-1, -1)):  # forced term-kernel schedules, then automatic
-            capi.check(capi.lib().dq_set_tuning(b"spmv_lds", lds))
+        # forced term-kernel schedules (LDS window, XCD order, rows per lane), then the timed choice
+        for lds, xcd, rpt in ((0, 0, 2), (1, 0, 4), (0, 1, 8), (1, 1, 2), (-1, -1, 0)):
+            capi.check(capi.lib().dq_set_tuning(b"hamsim_lds", lds))
             capi.check(capi.lib().dq_set_tuning(b"spmv_xcd", xcd))
+            capi.check(capi.lib().dq_set_tuning(b"hamsim_rows_per_lane", rpt))
             for t, r, m in ((0.5, 5, 12), (1.0, 3, 30)):
                 got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), t, r, m))
                 ref = O.hamiltonian_simulation(M, psi, t, r, m)
-                assert rel(got, ref) < TOL, (lds, xcd, t, r, m)
+                assert rel(got, ref) < TOL, (lds, xcd, rpt, t, r, m)
     finally:
-        capi.check(capi.lib().dq_set_tuning(b"spmv_lds", -1))
+        capi.check(capi.lib().dq_set_tuning(b"hamsim_lds", -1))
         capi.check(capi.lib().dq_set_tuning(b"spmv_xcd", -1))
+        capi.check(capi.lib().dq_set_tuning(b"hamsim_rows_per_lane", 0))
     exact = scipy.sparse.linalg.expm_multiply(-1j * 1.0 * O.to_dense(M), psi)
     got = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 10, 20))
     assert np.linalg.norm(got - exact) < 1e-10
     same = d.to_numpy_vector(d.hamiltonian_simulation_cuda(H, d.from_numpy_vector(psi), 1.0, 0))
     np.testing.assert_array_equal(same, psi)
+
+
+@pytest.mark.parametrize("model", ["xxz", "tfim"])
+def test_hamsim_zero_block_skip_is_exact(d, model):
+    """The fused evolution skips the 256-row blocks of a diagonal whose values are all zero (most of a qubit
+    Hamiltonian's flip diagonals). In FP64 a skipped block's products are exact zeros, so the result has the same bits
+    as reading every block, under every term-kernel schedule; in FP32 (compensated sums) it stays within FP32
+    rounding."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    n = 14
+    rows, cols, offs, vals = Hm.xxz(n, seed=5) if model == "xxz" else Hm.tfim(n, 1.0, 0.7)
+    psi = Hm.product_state(n, 6)
+    out = {}
+    try:
+        for prec in ("fp64", "fp32"):
+            D = d.to_device(dia_of(d, rows, cols, offs, vals), precision=prec)
+            dt = torch.float64 if prec == "fp64" else torch.float32
+            for skip in (1, 0):
+                for lds, rpt in ((0, 2), (1, 2), (0, 8), (1, 8), (-1, 0)):
+                    capi.check(L.dq_set_tuning(b"hamsim_skip_zero_blocks", skip))
+                    capi.check(L.dq_set_tuning(b"hamsim_lds", lds))
+                    capi.check(L.dq_set_tuning(b"hamsim_rows_per_lane", rpt))
+                    pr = torch.as_tensor(psi.real, device="cuda").to(dt).contiguous()
+                    pi = torch.as_tensor(psi.imag, device="cuda").to(dt).contiguous()
+                    d.hamsim_device(D, pr.data_ptr(), pi.data_ptr(), 0.8, 4, 14)
+                    torch.cuda.synchronize()
+                    out[prec, skip, lds, rpt] = (pr.double() + 1j * pi.double()).cpu().numpy()
+    finally:
+        capi.check(L.dq_set_tuning(b"hamsim_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"hamsim_lds", -1))
+        capi.check(L.dq_set_tuning(b"hamsim_rows_per_lane", 0))
+    base = out["fp64", 0, 0, 2]
+    for key, v in out.items():
+        if key[0] == "fp64":
+            np.testing.assert_array_equal(v, base, err_msg=str(key))
+        else:
+            assert rel(v, base) < 5e-6, key
+    M = O.Dia(rows, cols, offs, vals)
+    assert rel(out["fp64", 1, -1, 0], O.hamiltonian_simulation(M, psi, 0.8, 4, 14)) < TOL
+    for key, bad in ((b"hamsim_skip_zero_blocks", -1), (b"hamsim_rows_per_lane", 16), (b"hamsim_lds", 2)):
+        with pytest.raises(RuntimeError, match=key.decode()):
+            capi.check(L.dq_set_tuning(key, bad))
 
 
 def test_hamsim_device_in_place(d):
@@ -476,14 +523,14 @@ def test_hamsim_device_in_place(d):
     D32 = d.to_device(dia_of(d, rows, cols, offs, vals), precision="fp32")
     try:
         for lds in (0, 1):
-            capi.check(capi.lib().dq_set_tuning(b"spmv_lds", lds))
+            capi.check(capi.lib().dq_set_tuning(b"hamsim_lds", lds))
             qr = torch.as_tensor(psi.real, device="cuda").float().contiguous()
             qi = torch.as_tensor(psi.imag, device="cuda").float().contiguous()
             d.hamsim_device(D32, qr.data_ptr(), qi.data_ptr(), 0.6, 3, 15)
             torch.cuda.synchronize()
             assert rel((qr.double() + 1j * qi.double()).cpu().numpy(), got) < 5e-6, lds
     finally:
-        capi.check(capi.lib().dq_set_tuning(b"spmv_lds", -1))
+        capi.check(capi.lib().dq_set_tuning(b"hamsim_lds", -1))
 
 
 # ---------------------------------------------------------------------------------------------------------------------
