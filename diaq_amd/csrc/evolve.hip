@@ -171,7 +171,8 @@ __global__ __launch_bounds__(256) void probe_kernel(int64_t N, int n, const Diag
                                                     const float *__restrict__ p32r, const float *__restrict__ p32i,
                                                     const float *__restrict__ tin_r, const float *__restrict__ tin_i,
                                                     float *__restrict__ tout_r, float *__restrict__ tout_i,
-                                                    double *__restrict__ norms) {
+                                                    double *__restrict__ norms, const uint32_t *__restrict__ zmask,
+                                                    int64_t zwords) {
     const int j = blockIdx.y;
     const JobParam P = params[j];
     if (kstep > P.m + 1) return;
@@ -188,10 +189,17 @@ __global__ __launch_bounds__(256) void probe_kernel(int64_t N, int n, const Diag
         const DiagMeta md = meta[d];
         const int64_t lo = md.rlo, hi_ = md.rhi;
         if (hi_ <= tile || lo >= tile + RPP * RPT) continue;
+        uint32_t live = (1u << RPT) - 1;  // the zero-block map, as in horner_kernel
+        if (zmask) {
+            const int64_t b0 = static_cast<int64_t>(blockIdx.x) * RPT;
+            live = (zmask[(static_cast<int64_t>(j) * n + d) * zwords + (b0 >> 5)] >> (b0 & 31)) & ((1u << RPT) - 1);
+            if (!live) continue;  // uniform
+        }
         const TV *ar = hre + vb + md.rowbase;
         const TV *ai = him + vb + md.rowbase;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
+            if (!((live >> q) & 1u)) continue;  // uniform
             const int64_t r = tile + rl + RPP * q;
             const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);
             TV a_r = ar[rc], a_i = ai[rc];
@@ -435,7 +443,7 @@ static void launch_probe(const EvoClass &c, int kstep, hipStream_t s) {
     hipLaunchKernelGGL((probe_kernel<TV, NS, R>), grid, dim3(256), 0, s, N, (int)c.offs.size(),
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(),
                        kstep, c.p32r.as<float>(), c.p32i.as<float>(), tin_r, tin_i, tout_r, tout_i,
-                       c.d_probe.as<double>());
+                       c.d_probe.as<double>(), g_evo_skip ? c.zmask.as<uint32_t>() : nullptr, c.zwords);
     DQ_LAUNCH_CHECK();
 }
 
