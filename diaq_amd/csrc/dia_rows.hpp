@@ -97,8 +97,8 @@ __device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__res
 }
 
 // Zero-block map of a DIA layout: bit b of word [job][diagonal][b / 32] is set when some value of the diagonal in
-// rows [b * BR, (b + 1) * BR) is nonzero (a NaN counts as nonzero). One workgroup per (word, diagonal, job); the
-// values of job j start at j * total. A block whose bit is clear adds nothing to any row sum, so the row-tile loops
+// rows [b * BR, (b + 1) * BR) is nonzero (a NaN counts as nonzero). One workgroup per (word, diagonal, job), the
+// diagonals strided over gridDim.y (at most 65535); the values of job j start at j * total. A block whose bit is clear adds nothing to any row sum, so the row-tile loops
 // can skip its value loads and x gathers: a qubit Hamiltonian's +-2^k flip diagonals are zero on alternate runs of
 // 2^k rows (three runs in four for a flip-flop term), half or more of what they store.
 template <typename T, int BR>
@@ -107,26 +107,32 @@ __global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t rows, int n, c
                                                           const T *__restrict__ aim, int64_t zwords,
                                                           uint32_t *__restrict__ zmask) {
     __shared__ int live[32];
-    if (threadIdx.x < 32) live[threadIdx.x] = 0;
-    __syncthreads();
     const int64_t w = blockIdx.x;
-    const int d = blockIdx.y, j = blockIdx.z;
-    const DiagMeta md = meta[d];
-    const T *pr = are + static_cast<int64_t>(j) * total + md.rowbase;
-    const T *pi = aim + static_cast<int64_t>(j) * total + md.rowbase;
+    const int j = blockIdx.z;
     const int64_t r0 = w * 32 * BR;
-    for (int e = threadIdx.x; e < 32 * BR; e += 256) {
-        const int64_t r = r0 + e;
-        if (r >= md.rlo && r < md.rhi && r < rows && !(pr[r] == T(0) && pi[r] == T(0))) live[e / BR] = 1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t m = 0;
-        for (int b = 0; b < 32; ++b)
-            if (live[b]) m |= 1u << b;
-        zmask[(static_cast<int64_t>(j) * n + d) * zwords + w] = m;
+    for (int d = blockIdx.y; d < n; d += gridDim.y) {  // uniform
+        if (threadIdx.x < 32) live[threadIdx.x] = 0;
+        __syncthreads();
+        const DiagMeta md = meta[d];
+        const T *pr = are + static_cast<int64_t>(j) * total + md.rowbase;
+        const T *pi = aim + static_cast<int64_t>(j) * total + md.rowbase;
+        for (int e = threadIdx.x; e < 32 * BR; e += 256) {
+            const int64_t r = r0 + e;
+            if (r >= md.rlo && r < md.rhi && r < rows && !(pr[r] == T(0) && pi[r] == T(0))) live[e / BR] = 1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t m = 0;
+            for (int b = 0; b < 32; ++b)
+                if (live[b]) m |= 1u << b;
+            zmask[(static_cast<int64_t>(j) * n + d) * zwords + w] = m;
+        }
+        __syncthreads();  // live[] is cleared for the next diagonal
     }
 }
+
+// Grid rows for a zero-block map over n diagonals.
+inline unsigned zero_map_grid_y(int n) { return static_cast<unsigned>(n < 65535 ? n : 65535); }
 
 inline int64_t zero_block_words(int64_t rows, int block_rows) { return ((rows + block_rows - 1) / block_rows + 31) / 32; }
 

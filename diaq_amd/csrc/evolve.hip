@@ -517,6 +517,8 @@ extern "C" int dq_evo_create(int n_jobs, dq_devmat *const *H, int n_states, int 
             }
             E->where[j] = {ci, static_cast<int>(E->classes[ci].jobs.size())};
             E->classes[ci].jobs.push_back(j);
+            if (E->classes[ci].jobs.size() > 65535)  // a class's jobs index the grid's y (or z) dimension
+                throw ArgError("dq_evo_create: more than 65535 Hamiltonians of one size and diagonal set");
         }
         const size_t vb = elem_bytes(value_precision);
         for (auto &c : E->classes) {
@@ -562,7 +564,7 @@ extern "C" int dq_evo_create(int n_jobs, dq_devmat *const *H, int n_states, int 
             c.zwords = zero_block_words(c.N, static_cast<int>(rpp));
             c.zmask = DevBuf(sizeof(uint32_t) * std::max<int64_t>(1, c.zwords * nd * c.jobs_n()));
             if (nd == 0 || c.jobs_n() == 0) continue;
-            const dim3 grid(static_cast<unsigned>(c.zwords), static_cast<unsigned>(nd), static_cast<unsigned>(c.jobs_n()));
+            const dim3 grid(static_cast<unsigned>(c.zwords), zero_map_grid_y(nd), static_cast<unsigned>(c.jobs_n()));
             dispatch(value_precision, n_states, [&](auto tv, auto ns) {
                 using TV = decltype(tv);
                 constexpr int RPP = 256 / decltype(ns)::value;

@@ -197,7 +197,8 @@ static void evolve(const dq_devmat &H, T *pr, T *pi, double t, uint64_t r, int m
     DevBuf zmap;
     if (g_hamsim_skip && H.n() > 0 && m > 0) {
         zmap = DevBuf(sizeof(uint32_t) * zwords * H.n());
-        hipLaunchKernelGGL((zero_blocks_kernel<T, 256>), dim3(static_cast<unsigned>(zwords), H.n(), 1), dim3(256), 0,
+        hipLaunchKernelGGL((zero_blocks_kernel<T, 256>), dim3(static_cast<unsigned>(zwords), zero_map_grid_y(H.n()), 1),
+                           dim3(256), 0,
                            s, N, H.n(), H.d_meta.as<DiagMeta>(), int64_t(0), static_cast<const T *>(H.re.p),
                            static_cast<const T *>(H.im.p), zwords, zmap.as<uint32_t>());
         DQ_LAUNCH_CHECK();
