@@ -330,6 +330,54 @@ def evolution_cpu_baseline(torch, d, H, offs, vals, rows, psi0, dt, m, steps, si
                 gpu_vs_cpu_max_rel_diff=diff)
 
 
+def ntimes_section(args, torch, d, capi, Hm, n=24, n_times=10):
+    """spMV_ntimes on resident vectors for the 24-qubit TFIM itself (built on the device; its flip diagonals are zero
+    on alternate runs of 2^k rows): ten products per call, with the zero-block skip and without it. Bytes per product
+    count only the 256-row blocks holding a nonzero, plus x read and y written."""
+    import ctypes
+    x_m, z_m, c_m = Hm.pauli_masks(Hm.tfim_terms(n, 1.0, 0.7))
+    D = d.pauli_sum_device(n, x_m, z_m, c_m)
+    N = 1 << n
+    L = capi.lib()
+    best = ctypes.c_float()  # the plan step: SpMV schedules, the zero-block-skipping one included
+    capi.check(L.dq_spmv_tune(D.handle, 3, 0, ctypes.byref(best)))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    xr = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    xi = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    wr, wi = torch.empty_like(xr), torch.empty_like(xi)
+    sid = torch.cuda.current_stream().cuda_stream
+    out = dict(workload=f"spMV_ntimes, TFIM-{n} (J=1, h=0.7) as built on the device, {n_times} products per call, "
+                        "complex128, resident vectors")
+    res = {}
+    try:
+        for skip in (1, 0):
+            capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", skip))
+            d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times,
+                                 wr.data_ptr(), wi.data_ptr(), sid)  # warm
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times,
+                                 wr.data_ptr(), wi.data_ptr(), sid)
+            e1.record()
+            torch.cuda.synchronize()
+            res[skip] = (e0.elapsed_time(e1) / n_times, torch.complex(yr, yi))
+    finally:
+        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 1))
+    rows, cols, offs, vals = Hm.tfim(n, 1.0, 0.7)
+    live = live_stored(rows, cols, offs, vals, 256)
+    nbytes = 16 * live + 32 * N
+    same = bool(torch.equal(res[0][1], res[1][1]))
+    out.update(stored_elements=D.stored, live_stored_elements=live, bytes_per_product=nbytes,
+               ms_per_product=res[1][0], gbs=nbytes / (res[1][0] * 1e-3) / 1e9,
+               ms_per_product_reading_every_block=res[0][0], same_bits_with_and_without_skip=same)
+    log(f"[bench] spMV_ntimes TFIM-{n}: {res[1][0]:.3f} ms per product with the zero-block skip, {res[0][0]:.3f} ms "
+        f"without; {out['gbs']:.0f} GB/s of live bytes; same bits {same}")
+    return out
+
+
 def structured_spmv_section(args, torch, d, capi, Hm):
     """SpMV on a matrix of the headline's size with the offsets of a 24-qubit transverse-field Ising Hamiltonian
     (0 and +-2^k, k = 0..23; complex-normal values): what the LDS x window gains where diagonals share x."""
@@ -872,8 +920,8 @@ def main():
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
-                    choices=["spmv", "spmv_structured", "spgemm", "spmm_dia", "evolution", "batched", "conversion",
-                             "trotter", "builders"],
+                    choices=["spmv", "spmv_structured", "spmv_ntimes", "spgemm", "spmm_dia", "evolution", "batched",
+                             "conversion", "trotter", "builders"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -897,6 +945,9 @@ def main():
         elif args.profile == "spmv_structured":
             sp = structured_spmv_section(args, torch, d, capi, Hm)
             log(f"[bench] profile: structured spmv {sp['ms']:.3f} ms")
+        elif args.profile == "spmv_ntimes":
+            sp = ntimes_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: spMV_ntimes {sp['ms_per_product']:.3f} ms per product")
         elif args.profile == "spgemm":
             sg = spgemm_section(args, torch, d, capi, Hm, precisions=("fp64",))
             log(f"[bench] profile: spgemm {sg['fp64']['seconds'] * 1e3:.2f} ms")
@@ -933,7 +984,8 @@ def main():
             log(f"[bench] evolution failed: {e}")
     extra = {}
     if not args.no_evolution:
-        for key, fn in (("spmv_structured", structured_spmv_section), ("spgemm", spgemm_section),
+        for key, fn in (("spmv_structured", structured_spmv_section), ("spmv_ntimes", ntimes_section),
+                        ("spgemm", spgemm_section),
                         ("spmm_dia", spmm_section), ("expm", expm_section), ("conversion", conversion_section),
                         ("trotter", trotter_section), ("builders", builder_section)):
             try:
@@ -974,6 +1026,7 @@ def main():
         "spmv_vendor_csr": sp.get("vendor_csr"),
         "cpu_detail": cpu or None,
         "spmv_structured": extra.get("spmv_structured"),
+        "spmv_ntimes": extra.get("spmv_ntimes"),
         "evolution": evo,
         "spgemm": extra.get("spgemm"),
         "spmm_dia": extra.get("spmm_dia"),

@@ -144,6 +144,9 @@ struct dq_devmat {
     // Fused-evolution term schedule, timed on the first evolution of this matrix (0 = not yet): rows per lane (2, 4,
     // 8) and LDS x window (0, 1). A speed choice only: every schedule gives the same bits.
     mutable int term_rpt = 0, term_lds = 0;
+    // SpMV schedule for sequences that skip the matrix's all-zero blocks (spMV_ntimes), chosen by dq_spmv_tune
+    // (0 / -1 = the library default): rows per lane, non-temporal loads, load schedule, LDS x window.
+    int skip_rpt = 0, skip_nt = -1, skip_pipe = -1, skip_lds = -1;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {

@@ -36,13 +36,16 @@ static int g_spmv_pipe = -1;  // load schedule: -1 = automatic, 0 = per-row pred
 static int g_spmv_lds = -1;   // LDS x window: -1 = automatic, 0 = off, 1 = on
 static int g_spmv_xcd = -1;   // tile order: -1 = automatic, 0 = round-robin over the XCDs, 1 = XCD-contiguous
 
-template <typename T, int RPT, bool NT, bool PIPE, bool LDS>
+// ZM: zm is a zero-block map of A (256-row blocks, zwords words per diagonal) whose all-zero blocks are skipped;
+// only callers that hold A fixed for the map's lifetime pass one (spMV_ntimes).
+template <typename T, int RPT, bool NT, bool PIPE, bool LDS, bool ZM>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t cols, int n,
                                                         const DiagMeta *__restrict__ meta, const T *__restrict__ are,
                                                         const T *__restrict__ aim, const T *__restrict__ xr,
                                                         const T *__restrict__ xi, T *__restrict__ yr,
                                                         T *__restrict__ yi, int64_t col_lo, int64_t col_hi,
-                                                        int accumulate, int64_t ntiles, int xcd) {
+                                                        int accumulate, int64_t ntiles, int xcd,
+                                                        const uint32_t *__restrict__ zm, int64_t zwords) {
     constexpr int TILE = 256 * RPT;
     constexpr int HALO = LDS ? TILE / 2 : 0;
     constexpr int W = LDS ? TILE + 2 * HALO : 1;
@@ -60,7 +63,8 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t co
             if (r < rows) { ar[j].add(yr[r]); ai[j].add(yi[r]); }
         }
     }
-    dia_rows_tile<T, RPT, NT, PIPE, HALO>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai, sr, si);
+    dia_rows_tile<T, RPT, NT, PIPE, HALO, ZM>(tile, tile_end, n, meta, are, aim, xr, xi, col_lo, col_hi, ar, ai, sr,
+                                              si, zm, zwords);
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
         const int64_t r = tile + threadIdx.x + 256 * j;
@@ -70,16 +74,23 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t co
 
 template <typename T>
 static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *yi, int rpt, bool nt, bool pipe,
-                        bool lds, int xcd, int64_t c0, int64_t c1, int accumulate, hipStream_t s) {
+                        bool lds, int xcd, int64_t c0, int64_t c1, int accumulate, hipStream_t s,
+                        const uint32_t *zm = nullptr, int64_t zwords = 0) {
     const int64_t rows = A.rows, cols = A.cols;
     const DiagMeta *meta = A.d_meta.as<DiagMeta>();
     const T *are = static_cast<const T *>(A.re.p), *aim = static_cast<const T *>(A.im.p);
     const int64_t ntiles = (rows + 256 * rpt - 1) / (256 * rpt);
     const int64_t grid = grid_for_tiles(ntiles, xcd);
     if (grid > 0x7fffffffLL) throw ArgError("spmv: too many rows");
-#define DQ_SPMV_LAUNCH(R, NT, PIPE, LDS)                                                                         \
-    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE, LDS>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, \
-                       rows, cols, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate, ntiles, xcd)
+#define DQ_SPMV_LAUNCH_Z(R, NT, PIPE, LDS, Z)                                                                    \
+    hipLaunchKernelGGL((spmv_rows_kernel<T, R, NT, PIPE, LDS, Z>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, \
+                       s, rows, cols, A.n(), meta, are, aim, xr, xi, yr, yi, c0, c1, accumulate, ntiles, xcd, zm,    \
+                       zwords)
+#define DQ_SPMV_LAUNCH(R, NT, PIPE, LDS)                           \
+    do {                                                           \
+        if (zm) { DQ_SPMV_LAUNCH_Z(R, NT, PIPE, LDS, true); }      \
+        else { DQ_SPMV_LAUNCH_Z(R, NT, PIPE, LDS, false); }        \
+    } while (0)
 #define DQ_SPMV_SCHED(R, LDS)                                         \
     if (nt && pipe) DQ_SPMV_LAUNCH(R, true, true, LDS);               \
     else if (nt) DQ_SPMV_LAUNCH(R, true, false, LDS);                 \
@@ -99,6 +110,7 @@ static void launch_rows(const dq_devmat &A, const T *xr, const T *xi, T *yr, T *
         default: throw ArgError("spmv: rows per lane must be 1, 2, 4, 8 or 16");
     }
 #undef DQ_SPMV_LAUNCH
+#undef DQ_SPMV_LAUNCH_Z
 #undef DQ_SPMV_SCHED
 #undef DQ_SPMV_CASE
     DQ_LAUNCH_CHECK();
@@ -144,6 +156,25 @@ bool lds_window_pays(const dq_devmat &A, int rpt) {
     return near >= 2;
 }
 
+// The schedule of SpMVs that skip A's zero blocks: fewer live diagonals per tile favour smaller tiles and
+// per-row loads (measured on a 24-qubit TFIM: 4 rows per lane, per-row predicated non-temporal loads, LDS window).
+static int skip_rpt_for(const dq_devmat &A) {
+    if (g_spmv_rpt > 0) return g_spmv_rpt;
+    return A.skip_rpt > 0 ? A.skip_rpt : 4;
+}
+static bool skip_nt_for(const dq_devmat &A) {
+    if (g_spmv_nt >= 0) return g_spmv_nt == 1;
+    return A.skip_nt >= 0 ? A.skip_nt == 1 : true;
+}
+static bool skip_pipe_for(const dq_devmat &A) {
+    if (g_spmv_pipe >= 0) return g_spmv_pipe == 1;
+    return A.skip_pipe >= 0 ? A.skip_pipe == 1 : false;
+}
+static bool skip_lds_for(const dq_devmat &A, int rpt) {
+    if (g_spmv_lds >= 0) return g_spmv_lds == 1;
+    return A.skip_lds >= 0 ? A.skip_lds == 1 : lds_window_pays(A, rpt);
+}
+
 int xcd_tiles_for(const dq_devmat &A) {
     if (g_spmv_xcd >= 0) return g_spmv_xcd;
     if (A.tuned_xcd >= 0) return A.tuned_xcd;
@@ -159,7 +190,7 @@ bool lds_window_for(const dq_devmat &A, int rpt) {
 }
 
 void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, int n_bands,
-                 hipStream_t s) {
+                 hipStream_t s, const uint32_t *zm = nullptr, int64_t zwords = 0) {
     if (A.rows == 0) return;
     if (A.n() == 0) {  // no diagonals: y = 0 (a fill kernel, safe inside graph capture)
         const unsigned g = blocks_for(A.rows, 256, 4096);
@@ -174,23 +205,63 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     }
     if (n_bands <= 0) n_bands = auto_bands(A);
     if (n_bands > static_cast<int>(A.cols)) n_bands = std::max<int>(1, A.cols);
-    const int rpt = auto_rpt(A);
-    const bool nt = auto_nt(A), pipe = auto_pipe(A), lds = lds_window_for(A, rpt);
+    const int rpt = zm ? skip_rpt_for(A) : auto_rpt(A);
+    const bool nt = zm ? skip_nt_for(A) : auto_nt(A), pipe = zm ? skip_pipe_for(A) : auto_pipe(A),
+               lds = zm ? skip_lds_for(A, rpt) : lds_window_for(A, rpt);
     const int xcd = xcd_tiles_for(A);
     const int64_t cols = A.cols;
     for (int b = 0; b < n_bands; ++b) {
         const int64_t c0 = cols * b / n_bands, c1 = cols * (b + 1) / n_bands;
         if (A.precision == DQ_F64)
             launch_rows<double>(A, (const double *)xr, (const double *)xi, (double *)yr, (double *)yi, rpt, nt, pipe,
-                                lds, xcd, c0, c1, b > 0, s);
+                                lds, xcd, c0, c1, b > 0, s, zm, zwords);
         else
             launch_rows<float>(A, (const float *)xr, (const float *)xi, (float *)yr, (float *)yi, rpt, nt, pipe, lds,
-                               xcd, c0, c1, b > 0, s);
+                               xcd, c0, c1, b > 0, s, zm, zwords);
     }
 }
 
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s) {
     spmv_banded(A, xr, xi, yr, yi, 0, s);
+}
+
+static int g_ntimes_skip = 1;  // "spmv_ntimes_skip_zero_blocks"
+
+// Whether a sequence of n SpMVs on A's values should skip A's zero blocks: not for a single product (the map would
+// cost about as much as the product saves), nor with the skip switched off.
+static bool ntimes_uses_map(const dq_devmat &A, int n) {
+    return g_ntimes_skip && n >= 2 && A.n() > 0 && A.rows > 0;
+}
+
+static size_t ntimes_map_bytes(const dq_devmat &A, int64_t &zwords) {
+    zwords = zero_block_words(A.rows, 256);
+    return sizeof(uint32_t) * static_cast<size_t>(zwords) * static_cast<size_t>(A.n());
+}
+
+// Whether the map z of A shows a 256-row block, inside some diagonal's rows, whose values are all zero.
+static bool has_zero_blocks(const dq_devmat &A, const uint32_t *z, int64_t zwords, hipStream_t s) {
+    std::vector<uint32_t> h(static_cast<size_t>(zwords) * A.n());
+    DQ_HIP(hipMemcpyAsync(h.data(), z, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    DQ_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < A.n(); ++k) {
+        const DiagMeta &m = A.meta[k];
+        if (m.rhi <= m.rlo) continue;
+        for (int64_t b = m.rlo / 256; b <= (m.rhi - 1) / 256; ++b)
+            if (!((h[static_cast<size_t>(k) * zwords + (b >> 5)] >> (b & 31)) & 1u)) return true;
+    }
+    return false;
+}
+
+// A's zero-block map (256-row blocks) into z.
+static void ntimes_zero_map(const dq_devmat &A, int64_t zwords, uint32_t *z, hipStream_t s) {
+    const dim3 grid(static_cast<unsigned>(zwords), zero_map_grid_y(A.n()), 1);
+    if (A.precision == DQ_F64)
+        hipLaunchKernelGGL((zero_blocks_kernel<double, 256>), grid, dim3(256), 0, s, static_cast<int64_t>(A.rows),
+                           A.n(), A.d_meta.as<DiagMeta>(), int64_t(0), A.re.as<double>(), A.im.as<double>(), zwords, z);
+    else
+        hipLaunchKernelGGL((zero_blocks_kernel<float, 256>), grid, dim3(256), 0, s, static_cast<int64_t>(A.rows),
+                           A.n(), A.d_meta.as<DiagMeta>(), int64_t(0), A.re.as<float>(), A.im.as<float>(), zwords, z);
+    DQ_LAUNCH_CHECK();
 }
 
 }  // namespace dq
@@ -223,6 +294,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "alloc_cache_mb") {
             if (value < -1) throw ArgError("alloc_cache_mb must be -1 (an eighth of the device memory) or >= 0");
             set_alloc_cache_budget(value);
+        } else if (k == "spmv_ntimes_skip_zero_blocks") {
+            if (value != 0 && value != 1) throw ArgError("spmv_ntimes_skip_zero_blocks must be 0 or 1");
+            g_ntimes_skip = value;
         } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value) &&
                    !dense_set_tuning(k, value) && !convert_set_tuning(k, value) && !gates_set_tuning(k, value) &&
                    !hamsim_set_tuning(k, value)) {
@@ -252,10 +326,12 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
         const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe, saved_lds = g_spmv_lds,
                   saved_xcd = g_spmv_xcd;
         g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1; g_spmv_lds = -1; g_spmv_xcd = -1;
+        const uint32_t *tz = nullptr;  // the zero-block map while the skip schedule is timed
+        int64_t tzw = 0;
         auto time_it = [&]() {
-            spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);  // warm
+            spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw);  // warm
             DQ_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s);
+            for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw);
             DQ_HIP(hipEventRecord(e1, s));
             DQ_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -277,6 +353,29 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
             A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds; A->tuned_xcd = 1;
             const float ms = time_it();
             if (ms < best) { best = ms; bxcd = 1; }
+            A->tuned_xcd = bxcd;
+            // third stage: the schedule of sequences that skip A's all-zero blocks (spMV_ntimes), when it has any
+            A->skip_rpt = brpt; A->skip_nt = bnt; A->skip_pipe = bpipe; A->skip_lds = blds;
+            if (g_ntimes_skip) {
+                DevBuf zmap(ntimes_map_bytes(*A, tzw));
+                ntimes_zero_map(*A, tzw, zmap.as<uint32_t>(), s);
+                if (has_zero_blocks(*A, zmap.as<uint32_t>(), tzw, s)) {
+                    tz = zmap.as<uint32_t>();
+                    float zbest = 1e30f;
+                    int zr = 4, znt = 1, zp = 0, zl = 0;
+                    for (int lds = 0; lds <= 1; ++lds)
+                        for (int pipe = 0; pipe <= 1; ++pipe)
+                            for (int rpt : {1, 2, 4, 8, 16})
+                                for (int nt = 0; nt <= 1; ++nt) {
+                                    if (lds && !lds_window_pays(*A, rpt)) continue;
+                                    A->skip_rpt = rpt; A->skip_nt = nt; A->skip_pipe = pipe; A->skip_lds = lds;
+                                    const float zms = time_it();
+                                    if (zms < zbest) { zbest = zms; zr = rpt; znt = nt; zp = pipe; zl = lds; }
+                                }
+                    tz = nullptr;
+                    A->skip_rpt = zr; A->skip_nt = znt; A->skip_pipe = zp; A->skip_lds = zl;
+                }
+            }
         } catch (...) {
             g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
             g_spmv_xcd = saved_xcd;
@@ -396,14 +495,30 @@ extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const v
         // destinations alternate between y and the work vectors so that the last product lands in y; with x == y and
         // an odd count the first product must not overwrite x, so the order flips and a copy ends the sequence
         const bool flip = in_place && (n % 2 == 1);
+        // the zero blocks of A, skipped by every product of the sequence (A cannot change during the call). The map
+        // lives in stream-ordered memory, so the call stays asynchronous; while the stream is being captured into a
+        // graph no map is made (a replay must not read memory this call has given back).
+        int64_t zwords = 0;
+        uint32_t *zmap = nullptr;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        DQ_HIP(hipStreamIsCapturing(s, &cap));
+        if (ntimes_uses_map(*A, n) && cap == hipStreamCaptureStatusNone) {
+            if (hipMallocAsync(reinterpret_cast<void **>(&zmap), ntimes_map_bytes(*A, zwords), s) != hipSuccess) {
+                (void)hipGetLastError();  // no stream-ordered memory: run without the map
+                zmap = nullptr;
+            } else {
+                ntimes_zero_map(*A, zwords, zmap, s);
+            }
+        }
         const void *sr = xr, *si = xi;
         for (int t = 0; t < n; ++t) {
             const bool to_y = (((n - 1 - t) % 2) == 0) != flip;
             void *dr = to_y ? yr : work_r, *di = to_y ? yi : work_i;
-            spmv(*A, sr, si, dr, di, s);
+            spmv_banded(*A, sr, si, dr, di, 0, s, zmap, zwords);
             sr = dr;
             si = di;
         }
+        if (zmap) DQ_HIP(hipFreeAsync(zmap, s));
         if (flip) {
             DQ_HIP(hipMemcpyAsync(yr, work_r, M * eb, hipMemcpyDeviceToDevice, s));
             DQ_HIP(hipMemcpyAsync(yi, work_i, M * eb, hipMemcpyDeviceToDevice, s));
@@ -433,9 +548,15 @@ extern "C" int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const d
             throw ArgError("spmv_ntimes_cuda: repeated SpMV requires square matrix (M==N) for num_times > 1");
         DevVec a(N, A->precision), b(M, A->precision);
         a.put(xr, xi);
+        int64_t zwords = 0;
+        DevBuf zmap;
+        if (ntimes_uses_map(*A, n)) {
+            zmap = DevBuf(ntimes_map_bytes(*A, zwords));
+            ntimes_zero_map(*A, zwords, zmap.as<uint32_t>(), 0);
+        }
         DevVec *in = &a, *out = &b;
         for (int t = 0; t < n; ++t) {
-            spmv(*A, in->r.p, in->i.p, out->r.p, out->i.p, 0);
+            spmv_banded(*A, in->r.p, in->i.p, out->r.p, out->i.p, 0, 0, zmap.as<uint32_t>(), zwords);
             if (t != n - 1) std::swap(in, out);
         }
         out->get(yr, yi);

@@ -121,13 +121,17 @@ int dq_spmv_device(const dq_devmat *A, const void *xr_dev, const void *xi_dev, v
  * (src/spMV_gpu.cu:171-183, :200-204). */
 int dq_spmv_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi);
 
-/* y = A^n x for host vectors, by n SpMVs on the device. Replaces spMV_ntimes_cuda (src/spMV_ntimes_gpu.cu:61-263). */
+/* y = A^n x for host vectors, by n SpMVs on the device. Replaces spMV_ntimes_cuda (src/spMV_ntimes_gpu.cu:61-263).
+ * For n >= 2 the products skip A's all-zero 256-row blocks (a map made once per call; the same bits in FP64). */
 int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi, int n);
 
 /* y = A^num_times x on device vectors of A's precision, asynchronous on `stream`: the reference's spMV_ntimes
  * (src/spMV_ntimes_gpu.cu:61-263, same errors and num_times == 0 / no-diagonal rules) on data that stays resident.
  * work_r / work_i are one more vector of A.rows elements, needed when num_times > 1 or x == y (in place: the result
- * overwrites x); the products alternate between y and the work vectors. */
+ * overwrites x); the products alternate between y and the work vectors. For num_times >= 2 the products skip A's
+ * all-zero 256-row blocks through a map made once per call in stream-ordered memory (hipMallocAsync), with the
+ * schedule dq_spmv_tune chose for that; while `stream` is being captured into a graph no map is made. The FP64 result
+ * has the same bits either way. */
 int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int num_times,
                           void *work_r, void *work_i, uintptr_t stream);
 
@@ -137,8 +141,9 @@ int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi
                           int n_bands, uintptr_t stream);
 
 /* SpMV autotuner: times every (load schedule, rows per lane, non-temporal loads, LDS x window) candidate `repeats`
- * times on scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A (and by the
- * fused evolution with A). best_ms (may be NULL) receives its time. No reference counterpart. */
+ * times on scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A. When A has
+ * all-zero 256-row blocks it also times the candidates that skip them and keeps that winner for spMV_ntimes.
+ * best_ms (may be NULL) receives the plain SpMV's time. No reference counterpart. */
 int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 
 /* The SpMV schedule A currently uses (after tuning or the defaults); any output pointer may be NULL. No reference
@@ -162,7 +167,8 @@ int dq_release_cached_memory(void);
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule,
  * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
- * "hamsim_skip_zero_blocks" (1 = default, 0) does the same for the fused evolution's Taylor terms, whose schedule
+ * "spmv_ntimes_skip_zero_blocks" (1 = default, 0) and "hamsim_skip_zero_blocks" (1 = default, 0) do the same for
+ * spMV_ntimes and for the fused evolution's Taylor terms, whose schedule
  * "hamsim_rows_per_lane" (0 = timed per matrix, or 2, 4, 8) and "hamsim_lds" (-1 = timed, 0, 1) override;
  * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
  * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)

@@ -819,3 +819,76 @@ def test_spmv_ntimes_device_resident_and_in_place(d, n_times):
     if n_times == 0:
         with pytest.raises(RuntimeError, match="requires square"):
             d.spmv_ntimes_device(R, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 0)
+
+
+@pytest.mark.parametrize("model", ["tfim", "xxz"])
+def test_spmv_ntimes_zero_block_skip_is_exact(d, model):
+    """spMV_ntimes maps its matrix's all-zero 256-row blocks once per call and every product skips them. The skipped
+    products are exact zeros, so in FP64 the result has the same bits with the skip on or off, under forced SpMV
+    schedules, on resident vectors (in place too) and through the host-vector entry point; a graph-captured call
+    (which runs without the map) replays to the same bits."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    q = 14 if model == "tfim" else 13
+    rows, cols, offs, vals = Hm.tfim(q, 1.0, 0.7) if model == "tfim" else Hm.xxz(q, seed=4)
+    H = dia_of(d, rows, cols, offs, vals)
+    D = d.to_device(H)
+    rng = np.random.default_rng(50)
+    x = rng.standard_normal(rows) + 1j * rng.standard_normal(rows)
+    x /= np.linalg.norm(x)
+    ref = O.spmv_ntimes(O.Dia(rows, cols, offs, vals), x, 3)
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def device_run(n_times, in_place):
+        xr = torch.tensor(x.real, device="cuda")
+        xi = torch.tensor(x.imag, device="cuda")
+        wr, wi = torch.empty_like(xr), torch.empty_like(xi)
+        yr, yi = (xr, xi) if in_place else (torch.empty_like(xr), torch.empty_like(xi))
+        d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times, wr.data_ptr(),
+                             wi.data_ptr(), sid)
+        torch.cuda.synchronize()
+        return yr.cpu().numpy() + 1j * yi.cpu().numpy()
+
+    out = {}
+    try:
+        for skip in (0, 1):
+            capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", skip))
+            for lds, rpt in ((0, 1), (1, 4), (0, 16), (-1, 0)):
+                capi.check(L.dq_set_tuning(b"spmv_lds", lds))
+                capi.check(L.dq_set_tuning(b"spmv_rows_per_lane", rpt))
+                out[skip, lds, rpt, "dev3"] = device_run(3, False)
+                out[skip, lds, rpt, "dev3_in_place"] = device_run(3, True)
+            out[skip, "host3"] = d.to_numpy_vector(d.spMV_ntimes(H, d.from_numpy_vector(x), 3))
+        capi.check(L.dq_set_tuning(b"spmv_lds", -1))
+        capi.check(L.dq_set_tuning(b"spmv_rows_per_lane", 0))
+        capi.check(L.dq_spmv_tune(D.handle, 2, 0, None))  # also times the skip schedule (the TFIM has zero blocks)
+        out["tuned", "dev3"] = device_run(3, False)
+        out["tuned", "dev3_in_place"] = device_run(3, True)
+    finally:
+        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"spmv_lds", -1))
+        capi.check(L.dq_set_tuning(b"spmv_rows_per_lane", 0))
+    base = out[0, -1, 0, "dev3"]
+    for key, v in out.items():
+        np.testing.assert_array_equal(v, base, err_msg=str(key))
+    assert rel(base, ref) < TOL
+    # graph capture: the map is not made while capturing; every replay gives the eager bits
+    xr = torch.tensor(x.real, device="cuda")
+    xi = torch.tensor(x.imag, device="cuda")
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    wr, wi = torch.empty_like(xr), torch.empty_like(xi)
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=cs):
+        d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 3, wr.data_ptr(),
+                             wi.data_ptr(), cs.cuda_stream)
+    for _ in range(3):
+        yr.fill_(7.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(yr.cpu().numpy() + 1j * yi.cpu().numpy(), base)
+    with pytest.raises(RuntimeError, match="spmv_ntimes_skip_zero_blocks"):
+        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 3))
