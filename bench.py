@@ -353,7 +353,7 @@ def ntimes_section(args, torch, d, capi, Hm, n=24, n_times=10):
     res = {}
     try:
         for skip in (1, 0):
-            capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", skip))
+            capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", skip))
             d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), n_times,
                                  wr.data_ptr(), wi.data_ptr(), sid)  # warm
             torch.cuda.synchronize()
@@ -365,7 +365,7 @@ def ntimes_section(args, torch, d, capi, Hm, n=24, n_times=10):
             torch.cuda.synchronize()
             res[skip] = (e0.elapsed_time(e1) / n_times, torch.complex(yr, yi))
     finally:
-        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", 1))
     rows, cols, offs, vals = Hm.tfim(n, 1.0, 0.7)
     live = live_stored(rows, cols, offs, vals, 256)
     nbytes = 16 * live + 32 * N

@@ -632,6 +632,7 @@ extern "C" int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, con
         if (A->n() && (!re_dev || !im_dev)) throw ArgError("dq_devmat_set_packed_device: NULL device array");
         hipStream_t s = as_stream(stream);
         const size_t eb = elem_bytes(A->precision);
+        A->zmap_valid = false;  // the values change: the zero-block map of the last tuning no longer holds
         int64_t src = 0;
         for (int k = 0; k < A->n(); ++k) {
             if (A->len[k]) {

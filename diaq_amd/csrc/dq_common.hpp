@@ -144,9 +144,14 @@ struct dq_devmat {
     // Fused-evolution term schedule, timed on the first evolution of this matrix (0 = not yet): rows per lane (2, 4,
     // 8) and LDS x window (0, 1). A speed choice only: every schedule gives the same bits.
     mutable int term_rpt = 0, term_lds = 0;
-    // SpMV schedule for sequences that skip the matrix's all-zero blocks (spMV_ntimes), chosen by dq_spmv_tune
-    // (0 / -1 = the library default): rows per lane, non-temporal loads, load schedule, LDS x window.
+    // SpMV schedule for products that skip the matrix's all-zero blocks, chosen by dq_spmv_tune (0 / -1 = the
+    // library default): rows per lane, non-temporal loads, load schedule, LDS x window.
     int skip_rpt = 0, skip_nt = -1, skip_pipe = -1, skip_lds = -1;
+    // The zero-block map (256-row blocks, zmap_words words per diagonal) dq_spmv_tune keeps when the matrix has
+    // all-zero blocks; zmap_valid is cleared by every call that rewrites the values (dq_devmat_set_packed_device).
+    dq::DevBuf zmap;
+    int64_t zmap_words = 0;
+    bool zmap_valid = false;
 
     int n() const { return static_cast<int>(offs.size()); }
     int64_t stored() const {

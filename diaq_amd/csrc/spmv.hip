@@ -221,16 +221,20 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     }
 }
 
-void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s) {
-    spmv_banded(A, xr, xi, yr, yi, 0, s);
+static int g_spmv_skip = 1;  // "spmv_skip_zero_blocks"
+
+// Whether a sequence of n SpMVs on A's values should make its own zero-block map: not for a single product (the map
+// would cost about as much as the product saves), not when the handle already holds a valid one, nor with the skip
+// switched off.
+static bool ntimes_uses_map(const dq_devmat &A, int n) {
+    return g_spmv_skip && n >= 2 && A.n() > 0 && A.rows > 0 && !A.zmap_valid;
 }
 
-static int g_ntimes_skip = 1;  // "spmv_ntimes_skip_zero_blocks"
-
-// Whether a sequence of n SpMVs on A's values should skip A's zero blocks: not for a single product (the map would
-// cost about as much as the product saves), nor with the skip switched off.
-static bool ntimes_uses_map(const dq_devmat &A, int n) {
-    return g_ntimes_skip && n >= 2 && A.n() > 0 && A.rows > 0;
+// The map a product with A may use without making one: the handle's, for FP64 matrices. (In FP64 a skipped block
+// changes no bit; a compensated FP32 sum would change in its last bits, and a single FP32 SpMV keeps the same bits
+// whether or not A was tuned.)
+static const uint32_t *handle_map(const dq_devmat &A) {
+    return (g_spmv_skip && A.zmap_valid && A.precision == DQ_F64) ? A.zmap.as<uint32_t>() : nullptr;
 }
 
 static size_t ntimes_map_bytes(const dq_devmat &A, int64_t &zwords) {
@@ -264,6 +268,13 @@ static void ntimes_zero_map(const dq_devmat &A, int64_t zwords, uint32_t *z, hip
     DQ_LAUNCH_CHECK();
 }
 
+
+// y = A x: the schedule of A, skipping its all-zero blocks when the handle holds their map (see handle_map).
+void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s) {
+    const uint32_t *zm = handle_map(A);
+    spmv_banded(A, xr, xi, yr, yi, 0, s, zm, zm ? A.zmap_words : 0);
+}
+
 }  // namespace dq
 
 using namespace dq;
@@ -294,9 +305,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
         } else if (k == "alloc_cache_mb") {
             if (value < -1) throw ArgError("alloc_cache_mb must be -1 (an eighth of the device memory) or >= 0");
             set_alloc_cache_budget(value);
-        } else if (k == "spmv_ntimes_skip_zero_blocks") {
-            if (value != 0 && value != 1) throw ArgError("spmv_ntimes_skip_zero_blocks must be 0 or 1");
-            g_ntimes_skip = value;
+        } else if (k == "spmv_skip_zero_blocks") {
+            if (value != 0 && value != 1) throw ArgError("spmv_skip_zero_blocks must be 0 or 1");
+            g_spmv_skip = value;
         } else if (!product_set_tuning(k, value) && !evo_set_tuning(k, value) && !spmm_set_tuning(k, value) &&
                    !dense_set_tuning(k, value) && !convert_set_tuning(k, value) && !gates_set_tuning(k, value) &&
                    !hamsim_set_tuning(k, value)) {
@@ -354,13 +365,18 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
             const float ms = time_it();
             if (ms < best) { best = ms; bxcd = 1; }
             A->tuned_xcd = bxcd;
-            // third stage: the schedule of sequences that skip A's all-zero blocks (spMV_ntimes), when it has any
+            // third stage: when A has all-zero blocks, keep their map in the handle and time the schedules that
+            // skip them (used by A's FP64 SpMVs and by spMV_ntimes)
             A->skip_rpt = brpt; A->skip_nt = bnt; A->skip_pipe = bpipe; A->skip_lds = blds;
-            if (g_ntimes_skip) {
-                DevBuf zmap(ntimes_map_bytes(*A, tzw));
-                ntimes_zero_map(*A, tzw, zmap.as<uint32_t>(), s);
-                if (has_zero_blocks(*A, zmap.as<uint32_t>(), tzw, s)) {
-                    tz = zmap.as<uint32_t>();
+            A->zmap_valid = false;
+            if (g_spmv_skip) {
+                const size_t zb = ntimes_map_bytes(*A, tzw);
+                if (A->zmap.bytes != zb) A->zmap = DevBuf(zb);
+                A->zmap_words = tzw;
+                ntimes_zero_map(*A, tzw, A->zmap.as<uint32_t>(), s);
+                if (has_zero_blocks(*A, A->zmap.as<uint32_t>(), tzw, s)) {
+                    A->zmap_valid = true;
+                    tz = A->zmap.as<uint32_t>();
                     float zbest = 1e30f;
                     int zr = 4, znt = 1, zp = 0, zl = 0;
                     for (int lds = 0; lds <= 1; ++lds)
@@ -495,26 +511,30 @@ extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const v
         // destinations alternate between y and the work vectors so that the last product lands in y; with x == y and
         // an odd count the first product must not overwrite x, so the order flips and a copy ends the sequence
         const bool flip = in_place && (n % 2 == 1);
-        // the zero blocks of A, skipped by every product of the sequence (A cannot change during the call). The map
-        // lives in stream-ordered memory, so the call stays asynchronous; while the stream is being captured into a
-        // graph no map is made (a replay must not read memory this call has given back).
+        // the zero blocks of A, skipped by every product of the sequence (A cannot change during the call): the map
+        // dq_spmv_tune kept in the handle, or else, for two or more products, one made here in stream-ordered memory,
+        // so the call stays asynchronous. While the stream is being captured into a graph no map is made (a replay
+        // must not read memory this call has given back).
         int64_t zwords = 0;
         uint32_t *zmap = nullptr;
+        const uint32_t *zuse = n == 1 ? handle_map(*A) : ((g_spmv_skip && A->zmap_valid) ? A->zmap.as<uint32_t>() : nullptr);
+        if (zuse) zwords = A->zmap_words;
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         DQ_HIP(hipStreamIsCapturing(s, &cap));
-        if (ntimes_uses_map(*A, n) && cap == hipStreamCaptureStatusNone) {
+        if (!zuse && ntimes_uses_map(*A, n) && cap == hipStreamCaptureStatusNone) {
             if (hipMallocAsync(reinterpret_cast<void **>(&zmap), ntimes_map_bytes(*A, zwords), s) != hipSuccess) {
                 (void)hipGetLastError();  // no stream-ordered memory: run without the map
                 zmap = nullptr;
             } else {
                 ntimes_zero_map(*A, zwords, zmap, s);
+                zuse = zmap;
             }
         }
         const void *sr = xr, *si = xi;
         for (int t = 0; t < n; ++t) {
             const bool to_y = (((n - 1 - t) % 2) == 0) != flip;
             void *dr = to_y ? yr : work_r, *di = to_y ? yi : work_i;
-            spmv_banded(*A, sr, si, dr, di, 0, s, zmap, zwords);
+            spmv_banded(*A, sr, si, dr, di, 0, s, zuse, zuse ? zwords : 0);
             sr = dr;
             si = di;
         }
@@ -550,13 +570,16 @@ extern "C" int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const d
         a.put(xr, xi);
         int64_t zwords = 0;
         DevBuf zmap;
-        if (ntimes_uses_map(*A, n)) {
+        const uint32_t *zuse = n == 1 ? handle_map(*A) : ((g_spmv_skip && A->zmap_valid) ? A->zmap.as<uint32_t>() : nullptr);
+        if (zuse) zwords = A->zmap_words;
+        if (!zuse && ntimes_uses_map(*A, n)) {
             zmap = DevBuf(ntimes_map_bytes(*A, zwords));
             ntimes_zero_map(*A, zwords, zmap.as<uint32_t>(), 0);
+            zuse = zmap.as<uint32_t>();
         }
         DevVec *in = &a, *out = &b;
         for (int t = 0; t < n; ++t) {
-            spmv_banded(*A, in->r.p, in->i.p, out->r.p, out->i.p, 0, 0, zmap.as<uint32_t>(), zwords);
+            spmv_banded(*A, in->r.p, in->i.p, out->r.p, out->i.p, 0, 0, zuse, zuse ? zwords : 0);
             if (t != n - 1) std::swap(in, out);
         }
         out->get(yr, yi);

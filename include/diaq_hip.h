@@ -142,8 +142,9 @@ int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi
 
 /* SpMV autotuner: times every (load schedule, rows per lane, non-temporal loads, LDS x window) candidate `repeats`
  * times on scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A. When A has
- * all-zero 256-row blocks it also times the candidates that skip them and keeps that winner for spMV_ntimes.
- * best_ms (may be NULL) receives the plain SpMV's time. No reference counterpart. */
+ * all-zero 256-row blocks it also keeps their map in the handle and the fastest schedule that skips them: later FP64
+ * SpMVs with A and spMV_ntimes skip those blocks (the same bits; dq_devmat_set_packed_device drops the map). best_ms
+ * (may be NULL) receives the time of the schedule without the skip. No reference counterpart. */
 int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 
 /* The SpMV schedule A currently uses (after tuning or the defaults); any output pointer may be NULL. No reference
@@ -167,8 +168,8 @@ int dq_release_cached_memory(void);
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule,
  * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
- * "spmv_ntimes_skip_zero_blocks" (1 = default, 0) and "hamsim_skip_zero_blocks" (1 = default, 0) do the same for
- * spMV_ntimes and for the fused evolution's Taylor terms, whose schedule
+ * "spmv_skip_zero_blocks" (1 = default, 0) and "hamsim_skip_zero_blocks" (1 = default, 0) do the same for the
+ * SpMV (the map dq_spmv_tune keeps, and spMV_ntimes's own) and for the fused evolution's Taylor terms, whose schedule
  * "hamsim_rows_per_lane" (0 = timed per matrix, or 2, 4, 8) and "hamsim_lds" (-1 = timed, 0, 1) override;
  * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
  * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)

@@ -853,7 +853,7 @@ def test_spmv_ntimes_zero_block_skip_is_exact(d, model):
     out = {}
     try:
         for skip in (0, 1):
-            capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", skip))
+            capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", skip))
             for lds, rpt in ((0, 1), (1, 4), (0, 16), (-1, 0)):
                 capi.check(L.dq_set_tuning(b"spmv_lds", lds))
                 capi.check(L.dq_set_tuning(b"spmv_rows_per_lane", rpt))
@@ -866,7 +866,7 @@ def test_spmv_ntimes_zero_block_skip_is_exact(d, model):
         out["tuned", "dev3"] = device_run(3, False)
         out["tuned", "dev3_in_place"] = device_run(3, True)
     finally:
-        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", 1))
         capi.check(L.dq_set_tuning(b"spmv_lds", -1))
         capi.check(L.dq_set_tuning(b"spmv_rows_per_lane", 0))
     base = out[0, -1, 0, "dev3"]
@@ -890,5 +890,47 @@ def test_spmv_ntimes_zero_block_skip_is_exact(d, model):
         graph.replay()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(yr.cpu().numpy() + 1j * yi.cpu().numpy(), base)
-    with pytest.raises(RuntimeError, match="spmv_ntimes_skip_zero_blocks"):
-        capi.check(L.dq_set_tuning(b"spmv_ntimes_skip_zero_blocks", 3))
+    with pytest.raises(RuntimeError, match="spmv_skip_zero_blocks"):
+        capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", 3))
+
+
+def test_spmv_tuned_zero_block_map_is_exact_and_dropped_on_rewrite(d):
+    """dq_spmv_tune keeps the map of an FP64 matrix's all-zero 256-row blocks, and later SpMVs skip them: the same bits
+    as before tuning. Rewriting the values (dq_devmat_set_packed_device) drops the map, so the blocks that became
+    nonzero count again. An FP32 matrix keeps the same bits whether tuned or not (its SpMVs do not use the map)."""
+    import ctypes
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    rows, cols, offs, vals = Hm.tfim(13, 1.0, 0.7)
+    H = dia_of(d, rows, cols, offs, vals)
+    rng = np.random.default_rng(60)
+    x = rng.standard_normal(rows) + 1j * rng.standard_normal(rows)
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def spmv(D, dt):
+        xr = torch.as_tensor(x.real, device="cuda").to(dt).contiguous()
+        xi = torch.as_tensor(x.imag, device="cuda").to(dt).contiguous()
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        capi.check(L.dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
+        torch.cuda.synchronize()
+        return (yr.double() + 1j * yi.double()).cpu().numpy()
+
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        D = d.to_device(H, precision=prec)
+        before = spmv(D, dt)
+        capi.check(L.dq_spmv_tune(D.handle, 2, sid, None))
+        np.testing.assert_array_equal(spmv(D, dt), before, err_msg=prec)
+        if prec == "fp64":
+            assert rel(before, O.spmv(O.Dia(rows, cols, offs, vals), x)) < TOL
+            # new values, nonzero everywhere (the TFIM's zero blocks included)
+            S = D.stored
+            g = torch.Generator(device="cuda").manual_seed(61)
+            re = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+            im = torch.randn(S, dtype=torch.float64, device="cuda", generator=g)
+            capi.check(L.dq_devmat_set_packed_device(D.handle, re.data_ptr(), im.data_ptr(), sid))
+            torch.cuda.synchronize()
+            packed = (re + 1j * im).cpu().numpy()
+            lens = [len(v) for v in vals]
+            new_vals = np.split(packed, np.cumsum(lens)[:-1])
+            assert rel(spmv(D, dt), O.spmv(O.Dia(rows, cols, offs, new_vals), x)) < TOL
