@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -142,8 +143,9 @@ struct dq_devmat {
     // Row-tile order chosen by dq_spmv_tune (-1 = default round-robin, 0 = round-robin, 1 = XCD-contiguous).
     int tuned_xcd = -1;
     // Fused-evolution term schedule, timed on the first evolution of this matrix (0 = not yet): rows per lane (2, 4,
-    // 8) and LDS x window (0, 1). A speed choice only: every schedule gives the same bits.
-    mutable int term_rpt = 0, term_lds = 0;
+    // 8) and LDS x window (0, 1). A speed choice only: every schedule gives the same bits. Atomic: evolutions of one
+    // matrix may run from several threads, and the first ones may each time and store a choice.
+    mutable std::atomic<int> term_rpt{0}, term_lds{0};
     // SpMV schedule for products that skip the matrix's all-zero blocks, chosen by dq_spmv_tune (0 / -1 = the
     // library default): rows per lane, non-temporal loads, load schedule, LDS x window.
     int skip_rpt = 0, skip_nt = -1, skip_pipe = -1, skip_lds = -1;

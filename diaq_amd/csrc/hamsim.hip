@@ -156,11 +156,11 @@ static void term_schedule(const dq_devmat &H, const T *pr, const T *pi, T *outr,
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        H.term_lds = blds;  // before term_rpt, which marks the choice as made
         H.term_rpt = brpt;
-        H.term_lds = blds;
     }
-    rpt = g_hamsim_rpt ? g_hamsim_rpt : H.term_rpt;
-    lds = g_hamsim_lds >= 0 ? g_hamsim_lds == 1 : H.term_lds == 1;
+    rpt = g_hamsim_rpt ? g_hamsim_rpt : H.term_rpt.load();
+    lds = g_hamsim_lds >= 0 ? g_hamsim_lds == 1 : H.term_lds.load() == 1;
 }
 
 // One step out = sum_{k=0}^{m} (-i dt H)^k psi / k! (src/spHamSim_gpu_all_fused.cu:235-277). t0/t1: scratch terms.
