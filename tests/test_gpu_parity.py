@@ -934,3 +934,114 @@ def test_spmv_tuned_zero_block_map_is_exact_and_dropped_on_rewrite(d):
             lens = [len(v) for v in vals]
             new_vals = np.split(packed, np.cumsum(lens)[:-1])
             assert rel(spmv(D, dt), O.spmv(O.Dia(rows, cols, offs, new_vals), x)) < TOL
+
+
+def test_spmv_tuned_zero_block_map_rectangular_and_ragged(d):
+    """A rectangular matrix whose diagonals start and end inside 256-row blocks, with zeroed row and column ranges:
+    tuning keeps its zero-block map, and the FP64 SpMV after tuning has the same bits as before and matches the
+    oracle; so do spMV_ntimes-style sequences on a square part (host entry point)."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    rng = np.random.default_rng(70)
+    m, n = 3000, 5000
+    A = banded(m, [-2900, -300, -1, 0, 7, 700, 4999], rng, cols=n)
+    A[256:1100, :] = 0       # rows of several whole blocks, and parts of two more
+    A[:, 2000:2600] = 0      # a column range: different rows on each diagonal
+    A[2999, :] = 0
+    D = d.to_device(d.from_numpy(A))
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def spmv():
+        xr = torch.tensor(x.real, device="cuda")
+        xi = torch.tensor(x.imag, device="cuda")
+        yr = torch.empty(m, dtype=torch.float64, device="cuda")
+        yi = torch.empty_like(yr)
+        capi.check(L.dq_spmv_device(D.handle, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), sid))
+        torch.cuda.synchronize()
+        return yr.cpu().numpy() + 1j * yi.cpu().numpy()
+
+    before = spmv()
+    capi.check(L.dq_spmv_tune(D.handle, 2, sid, None))
+    np.testing.assert_array_equal(spmv(), before)
+    assert rel(before, A @ x) < TOL
+    S = A[:, :m].copy()  # a square part for repeated products
+    Hs = d.from_numpy(S)
+    xs = x[:m] / np.linalg.norm(x[:m])
+    got = {}
+    for skip in (0, 1):
+        capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", skip))
+        got[skip] = d.to_numpy_vector(d.spMV_ntimes(Hs, d.from_numpy_vector(xs), 4))
+    capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", 1))
+    np.testing.assert_array_equal(got[0], got[1])
+    assert rel(got[1], np.linalg.matrix_power(S, 4) @ xs) < TOL
+
+
+def test_reference_api_vs_numpy_script(d):
+    """python_tests/test_api_vs_numpy.py:12-92 through diaq_gpu: the same generator (seed 3), the same random
+    matrices drawn in the same order, and the same checks and bounds (max abs error / max(1, max |expected|) <= 1e-10,
+    expm 1e-8) for the functions this module has (round trip, product, spMV, trace, pow, the three Kronecker-times-
+    vector functions, expm, expm_negative). The draws for the host-only functions (add, adjoint, kroneckerProduct,
+    ...) still happen, so every later input is the reference's."""
+    import scipy.linalg
+    rng = np.random.default_rng(3)
+
+    def rand(r, c, n_diags=None, fill=0.7):
+        A = np.zeros((r, c), complex)
+        count = min(n_diags or rng.integers(1, 6), r + c - 1)
+        for o in rng.choice(np.arange(-(r - 1), c), size=count, replace=False):
+            k = np.arange(max(0, -o), min(r, c - o))
+            keep = rng.random(len(k)) < fill
+            A[k[keep], k[keep] + o] = rng.standard_normal(keep.sum()) + 1j * rng.standard_normal(keep.sum())
+        return A
+
+    worst = {}
+
+    def check(name, got, want, tol=1e-10):
+        got, want = np.asarray(got), np.asarray(want)
+        if got.shape != want.shape and got.size == want.size:
+            got = got.reshape(want.shape)
+        assert got.shape == want.shape and np.all(np.isfinite(got)), name
+        err = float(np.max(np.abs(got - want), initial=0.0)) / max(1.0, float(np.max(np.abs(want), initial=0.0)))
+        worst[name] = max(worst.get(name, 0.0), err)
+        assert err <= tol, (name, err)
+
+    M, N = d.from_numpy, lambda m: np.array(d.to_numpy(m))
+    V, NV = d.from_numpy_vector, lambda v: np.array(d.to_numpy_vector(v))
+    for _ in range(25):
+        r, c, k = (int(v) for v in rng.integers(2, 12, size=3))
+        A, _A2, B = rand(r, c), rand(r, c), rand(c, k)
+        S = rand(r, r)
+        x = rng.standard_normal(c) + 1j * rng.standard_normal(c)
+        check("round trip", N(M(A)), A)
+        check("product", N(d.product(M(A), M(B))), A @ B)
+        check("spMV", NV(d.spMV(M(A), V(x))), A @ x)
+        check("trace", d.trace(M(S)), np.trace(S))
+        check("pow 3", N(d.pow(M(S / 4), 3)), np.linalg.matrix_power(S / 4, 3))
+        check("pow 4", N(d.pow(M(S / 4), 4)), np.linalg.matrix_power(S / 4, 4))
+        rand(int(rng.integers(2, 5)), int(rng.integers(2, 5)))  # kroneckerProduct's operand (host-only function)
+        s1, s2 = (int(v) for v in rng.integers(0, 3, size=2))
+        I1, I2 = np.eye(1 << s1), np.eye(1 << s2)
+        Sq = rand(r, r)
+        big = np.kron(np.kron(I1, Sq), I2)
+        xb = rng.standard_normal(big.shape[1]) + 1j * rng.standard_normal(big.shape[1])
+        y = V(np.zeros(big.shape[0], complex))
+        check("identity_kron_matrix_kron_identity_times_vector",
+              NV(d.identity_kron_matrix_kron_identity_times_vector(s1, M(Sq), s2, V(xb), y)), big @ xb)
+        y = V(np.zeros(Sq.shape[0] << s1, complex))
+        xi = rng.standard_normal(Sq.shape[1] << s1) + 0j
+        check("identity_kron_matrix_times_vector", NV(d.identity_kron_matrix_times_vector(s1, M(Sq), V(xi), y)),
+              np.kron(I1, Sq) @ xi)
+        y = V(np.zeros(Sq.shape[0] << s2, complex))
+        xm = rng.standard_normal(Sq.shape[1] << s2) + 0j
+        check("matrix_kron_identity_times_vector", NV(d.matrix_kron_identity_times_vector(M(Sq), s2, V(xm), y)),
+              np.kron(Sq, I2) @ xm)
+    for _ in range(5):
+        n = int(rng.integers(2, 10))
+        H = rand(n, n, 3)
+        H = (H + H.conj().T) / 8
+        check("expm", N(d.expm(M(H), 100, 1.0, 1e-14)), scipy.linalg.expm(H), 1e-8)
+        check("expm_negative", N(d.expm_negative(M(H), 0.3, 100, 1.0, 1e-14)), scipy.linalg.expm(-1j * 0.3 * H),
+              1e-8)
+    assert len(worst) == 11  # every named check ran
