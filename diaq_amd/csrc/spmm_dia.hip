@@ -7,7 +7,7 @@
 // n_right, both row-major; double or interleaved complex double.
 //
 // Default mapping on gfx950 (spmm_dia_cols_kernel): a lane owns two adjacent columns of one row of C when n_right is
-// even, one element otherwise. One element per lane (spmm_dia_kernel): lane e owns C's flat element
+// even, one element otherwise; many rows with wide right-hand sides use the row-block form (spmm_dia_rows_kernel). One element per lane (spmm_dia_kernel): lane e owns C's flat element
 // e = r * n_right + j. Its B operand for diagonal k is
 // B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run of B and
 // writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles rows,
@@ -143,6 +143,70 @@ __global__ __launch_bounds__(256) void spmm_dia_cols_kernel(I n_rows, I n_cols, 
     }
 }
 
+// Row-block form for wide right-hand sides: a lane owns J adjacent columns of R consecutive rows of C. It keeps the
+// B rows its R rows need for the current diagonal in registers; when the next diagonal's offset is one more (a
+// band), its rows are the window shifted by one, so one new B row is loaded instead of R. Other offsets reload the
+// window. Each element adds its diagonals in slot order with the same fma_val sequence as the other forms.
+template <typename T, typename I, int J, int R>
+__global__ __launch_bounds__(256) void spmm_dia_rows_kernel(I n_rows, I n_cols, I n_right, int n_diags,
+                                                            const int64_t *__restrict__ offsets,
+                                                            const T *__restrict__ vals, const T *__restrict__ B,
+                                                            T *__restrict__ C) {
+    const I nchunk = n_right / J;
+    const I nblk = (n_rows + R - 1) / R;
+    const int64_t total = static_cast<int64_t>(nblk) * nchunk;
+    for (int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < total;
+         w += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t blk = w / nchunk;
+        const int64_t col = (w - blk * nchunk) * J;
+        const int64_t r0 = blk * R;
+        T acc[R][J], win[R][J];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int j = 0; j < J; ++j) acc[i][j] = zero_val<T>();
+        auto load_row = [&](int64_t b, T (&dst)[J]) {  // B row b (zero outside B)
+            const bool in = b >= 0 && b < static_cast<int64_t>(n_cols);
+            const T *bp = B + (in ? b : 0) * static_cast<int64_t>(n_right) + col;
+#pragma unroll
+            for (int j = 0; j < J; ++j) dst[j] = in ? bp[j] : zero_val<T>();
+        };
+        int64_t prev = 0;
+        for (int d = 0; d < n_diags; ++d) {
+            const int64_t o = offsets[d];  // uniform
+            if (d > 0 && o == prev + 1) {  // uniform: a band step, the window moves down one row
+#pragma unroll
+                for (int i = 0; i + 1 < R; ++i)
+#pragma unroll
+                    for (int j = 0; j < J; ++j) win[i][j] = win[i + 1][j];
+                load_row(r0 + R - 1 + o, win[R - 1]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < R; ++i) load_row(r0 + i + o, win[i]);
+            }
+            prev = o;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int64_t r = r0 + i;
+                const int64_t c = r + o;
+                if (r < static_cast<int64_t>(n_rows) && c >= 0 && c < static_cast<int64_t>(n_cols)) {
+                    const T a = ld_stream(vals + static_cast<int64_t>(d) * n_rows + r);
+#pragma unroll
+                    for (int j = 0; j < J; ++j) acc[i][j] = fma_val(acc[i][j], a, win[i][j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int64_t r = r0 + i;
+            if (r < static_cast<int64_t>(n_rows)) {
+#pragma unroll
+                for (int j = 0; j < J; ++j) st_stream(C + r * static_cast<int64_t>(n_right) + col + j, acc[i][j]);
+            }
+        }
+    }
+}
+
 // Tiled form: workgroup b owns rows [r0, r0 + TR) with TR = SPMM_TILE / n_right (SPMM_EPT elements per lane). When
 // two or more diagonals have |offset| <= TR/2 it first stages B's rows [r0 - TR/2, r0 + 3 TR/2) (zero outside B) in
 // LDS and those diagonals read B from there; the others read it from memory. Each element still sums its diagonals
@@ -231,6 +295,24 @@ static void launch_cols(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_d
 // fastest for every even n_right tried (8, 16, 32, 128: 10-22 % over one element per lane); 4 and 8 were slower (a
 // wave's loads then spread over 4-8 times more cache lines per instruction).
 static int g_spmm_cols = 0;
+// dq_set_tuning "spmm_rows_per_lane": rows per lane of the row-block form (spmm_dia_rows_kernel); -1 = automatic
+// (2 rows per lane for at least 32768 rows and an even n_right of at least 64, else off), 0 = off, 2, 4, 8. Measured
+// on 65536 rows x 128 columns: 1.34x for a 256-band, 1.11x for 256 scattered diagonals; with 16 columns, or 4096
+// rows (too few lanes), the flat forms are faster.
+static int g_spmm_rows = -1;
+
+template <typename T, int J, int R>
+static void launch_rows_block(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off,
+                              const T *vals, const T *B, T *C, bool small, hipStream_t s) {
+    const int64_t work = ((n_rows + R - 1) / R) * (n_right / J);
+    const unsigned grid = blocks_for(work, 256, 1 << 20);
+    if (small)
+        hipLaunchKernelGGL((spmm_dia_rows_kernel<T, uint32_t, J, R>), dim3(grid), dim3(256), 0, s, (uint32_t)n_rows,
+                           (uint32_t)n_cols, (uint32_t)n_right, n_diags, off, vals, B, C);
+    else
+        hipLaunchKernelGGL((spmm_dia_rows_kernel<T, int64_t, J, R>), dim3(grid), dim3(256), 0, s, n_rows, n_cols,
+                           n_right, n_diags, off, vals, B, C);
+}
 
 template <typename T>
 static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags, const int64_t *off, const T *vals,
@@ -238,6 +320,24 @@ static void launch(int64_t n_rows, int64_t n_cols, int64_t n_right, int n_diags,
     const int64_t total = n_rows * n_right;
     if (total == 0) return;
     const bool small = total < (int64_t(1) << 31) && n_cols < (int64_t(1) << 31);
+    const int rows_per_lane =
+        g_spmm_rows >= 0 ? g_spmm_rows : ((n_rows >= 32768 && n_right >= 64 && n_right % 2 == 0) ? 2 : 0);
+    if (ept == 0 && rows_per_lane > 0) {
+        const int J = n_right % 2 == 0 ? 2 : 1;
+        if (rows_per_lane == 8) {
+            if (J == 2) launch_rows_block<T, 2, 8>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+            else launch_rows_block<T, 1, 8>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        } else if (rows_per_lane == 4) {
+            if (J == 2) launch_rows_block<T, 2, 4>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+            else launch_rows_block<T, 1, 4>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        } else {
+            if (J == 2) launch_rows_block<T, 2, 2>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+            else launch_rows_block<T, 1, 2>(n_rows, n_cols, n_right, n_diags, off, vals, B, C, small, s);
+        }
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) throw HipError(std::string("spmm_dia_device: ") + hipGetErrorString(err));
+        return;
+    }
     if (ept == 0 && g_spmm_cols >= 0) {
         int J = 1;
         if (g_spmm_cols == 0) J = n_right % 2 == 0 ? 2 : 1;
@@ -290,6 +390,12 @@ bool spmm_set_tuning(const std::string &key, int value) {
         if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
             throw ArgError("spmm_cols_per_lane must be -1, 0, 1, 2, 4 or 8");
         g_spmm_cols = value;
+        return true;
+    }
+    if (key == "spmm_rows_per_lane") {
+        if (value != -1 && value != 0 && value != 2 && value != 4 && value != 8)
+            throw ArgError("spmm_rows_per_lane must be -1, 0, 2, 4 or 8");
+        g_spmm_rows = value;
         return true;
     }
     if (key != "spmm_tiled") return false;

@@ -172,7 +172,9 @@ int dq_release_cached_memory(void);
  * SpMV (the map dq_spmv_tune keeps, and spMV_ntimes's own) and for the fused evolution's Taylor terms, whose schedule
  * "hamsim_rows_per_lane" (0 = timed per matrix, or 2, 4, 8) and "hamsim_lds" (-1 = timed, 0, 1) override;
  * "spmm_tiled" (0 = not tiled, the default; 1, 2, 4 = LDS-tiled with that many elements per lane) and
- * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane)
+ * "spmm_cols_per_lane" (0 = automatic: 2 for an even n_right, else 1; 1, 2, 4, 8; -1 = one C element per lane),
+ * "spmm_rows_per_lane" (-1 = automatic: 2 for >= 32768 rows and an even n_right >= 64, else 0; 0 = off; 2, 4, 8: the
+ * row-block form, whose lane keeps a window of B rows that a band reuses)
  * pick the spmm_dia_device kernel; "product_dense" (0 = never, 1 = default: a near-dense product inside pow or the matrix
  * exponential goes through a dense library GEMM); "dense_convert" (0 = automatic, 1 = per-row kernels, 2 = tiled
  * kernels) picks the kernels of dq_devmat_from_dense_device / dq_devmat_to_dense_device; "dense_convert_tile" (0 =

@@ -561,7 +561,9 @@ def test_spmm_dia_device(d, case, complex_):
 
 # (spmm_tiled, spmm_cols_per_lane): the row-chunk kernel (automatic and each width), the one-element-per-lane
 # kernel, the LDS-tiled kernel with 1, 2, 4 elements per lane
-SPMM_KERNELS = [(0, 0), (0, -1), (0, 1), (0, 2), (0, 4), (0, 8), (1, 0), (2, 0), (4, 0)]
+# (spmm_tiled, spmm_cols_per_lane, spmm_rows_per_lane): every kernel form; -1 rows = the automatic choice
+SPMM_KERNELS = [(0, 0, 0), (0, -1, 0), (0, 1, 0), (0, 2, 0), (0, 4, 0), (0, 8, 0), (1, 0, 0), (2, 0, 0), (4, 0, 0),
+                (0, 0, 2), (0, 0, 4), (0, 0, 8), (0, 0, -1)]
 
 
 def spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, complex_, ref):
@@ -569,22 +571,24 @@ def spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, complex_, ref):
     from diaq_amd import capi
     first = None
     try:
-        for tiled, cols in SPMM_KERNELS:
+        for tiled, cols, rows in SPMM_KERNELS:
             capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", tiled))
             capi.check(capi.lib().dq_set_tuning(b"spmm_cols_per_lane", cols))
+            capi.check(capi.lib().dq_set_tuning(b"spmm_rows_per_lane", rows))
             tC.fill_(float("nan"))
             d.spmm_dia_device(off.data_ptr(), tV.data_ptr(), len(offs), M, K, tB.data_ptr(), R, tC.data_ptr(),
                               complex_, torch.cuda.current_stream().cuda_stream)
             torch.cuda.synchronize()
             got = tC.cpu().numpy()
-            assert rel(got, ref) < 1e-13, (tiled, cols)
+            assert rel(got, ref) < 1e-13, (tiled, cols, rows)
             if first is None:
                 first = got
             else:
-                assert np.array_equal(got, first), (tiled, cols)
+                assert np.array_equal(got, first), (tiled, cols, rows)
     finally:
         capi.check(capi.lib().dq_set_tuning(b"spmm_tiled", 0))
         capi.check(capi.lib().dq_set_tuning(b"spmm_cols_per_lane", 0))
+        capi.check(capi.lib().dq_set_tuning(b"spmm_rows_per_lane", -1))
 
 
 @pytest.mark.parametrize("M,K,R,offs", [
@@ -595,6 +599,8 @@ def spmm_every_kernel(d, torch, off, tV, tB, tC, offs, M, K, R, complex_, ref):
     (4096, 4096, 16, list(range(-20, 21))),            # 41 diagonals: several unrolled groups and a ragged one
     (1025, 1100, 128, [-1024, -3, 0, 9, 1099]),        # wide: 16 chunks of 8 columns per row
     (333, 333, 12, [-5, 1, 4]),                        # 12 columns: chunks of 4
+    (1001, 1003, 5, [-7, -6, -5, 10, 11, 30, 1002]),   # runs and jumps of offsets, odd columns, ragged row blocks
+    (40000, 40000, 64, list(range(-3, 4))),            # many rows, wide: the automatic row-block form
 ])
 def test_spmm_dia_device_tiling_edges(d, M, K, R, offs):
     import torch
