@@ -436,10 +436,10 @@ struct ViabilityRule {
 };
 
 // The reference's constants (an A100 sweep) and the ones fitted on MI355X by diaq_amd.viability_sweep over the same
-// 360-case grid: no case passing them ran slower in DIA than hipSPARSE's CSR x dense, and they keep 196 of the 293
+// 360-case grid: no case passing them ran slower in DIA than hipSPARSE's CSR x dense, and they keep 228 of the 305
 // cases where DIA was faster (the reference's constants keep 63).
 constexpr ViabilityRule REFERENCE_RULE{0.9, 16, 32, 32, 8};
-constexpr ViabilityRule MI355X_RULE{6.0, 64, 128, 16, 16};
+constexpr ViabilityRule MI355X_RULE{6.0, 64, 128, 16, 64};
 
 static dq_dia_viability_t viability(const ViabilityRule &R, int64_t n_rows, int64_t n_diags, int64_t nnz,
                                     int64_t n_right, int itemsize, int index_bytes) {

@@ -308,8 +308,8 @@ int dq_dia_viability(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nn
                      int index_bytes, dq_dia_viability_t *out);
 
 /* The same two budgets with constants fitted on MI355X over the reference's 360-case grid (spmm_dia_device against
- * hipSPARSE's CSR x dense, complex double): DIA bytes <= 6 x CSR bytes, n_diags <= max(64, n_rows / 128), capped at 16
- * when n_right >= 16. No case passing them ran slower in DIA; they keep 196 of the 293 DIA wins (the reference's
+ * hipSPARSE's CSR x dense, complex double): DIA bytes <= 6 x CSR bytes, n_diags <= max(64, n_rows / 128), capped at 64
+ * when n_right >= 16. No case passing them ran slower in DIA; they keep 228 of the 305 DIA wins (the reference's
  * constants keep 63). No reference counterpart (the reference's constants come from an A100). */
 int dq_dia_viability_mi355x(int64_t n_rows, int64_t n_cols, int64_t n_diags, int64_t nnz, int64_t n_right,
                             int itemsize, int index_bytes, dq_dia_viability_t *out);

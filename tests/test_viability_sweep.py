@@ -40,7 +40,7 @@ def test_library_mi355x_rule_is_the_fitted_constants():
     """dq_dia_viability_mi355x applies the constants the sweep fitted on MI355X (profiles/viability_sweep_*.txt)."""
     import ctypes
     from diaq_amd import capi
-    fitted = dict(max_bytes_over_csr=6.0, min_budget=64, rows_per_diagonal=128, wide_right=16, wide_budget=16)
+    fitted = dict(max_bytes_over_csr=6.0, min_budget=64, rows_per_diagonal=128, wide_right=16, wide_budget=64)
     rng = np.random.default_rng(5)
     for _ in range(300):
         n_rows = int(rng.choice([4096, 65536, 1000, 77]))
