@@ -468,7 +468,7 @@ extern "C" int dq_devmat_from_dense_device(uint32_t rows, uint32_t cols, const d
         std::vector<unsigned char> flags(nd);
         {
             DevBuf dflags(nd);
-            DQ_HIP(hipMemsetAsync(dflags.p, 0, nd, s));
+            zero_bytes(dflags.p, nd, s);
             hipLaunchKernelGGL(dense_nonzero_diags_kernel, dim3(tile_grid(ntiles)), dim3(256), 0, s,
                                static_cast<int64_t>(rows), static_cast<int64_t>(cols), dense, ld, tiles_c, ntiles,
                                dflags.as<unsigned char>());

@@ -129,8 +129,8 @@ std::unique_ptr<dq_devmat> product_dense(const dq_devmat &A, const dq_devmat &B,
     const int64_t M = A.rows, K = A.cols, N = B.cols;
     const size_t eb = 2 * static_cast<size_t>(elem_bytes(A.precision));  // complex element
     DevBuf Ad(static_cast<size_t>(M * K) * eb), Bd(static_cast<size_t>(K * N) * eb), Cd(static_cast<size_t>(M * N) * eb);
-    DQ_HIP(hipMemsetAsync(Ad.p, 0, Ad.bytes, s));
-    DQ_HIP(hipMemsetAsync(Bd.p, 0, Bd.bytes, s));
+    zero_bytes(Ad.p, Ad.bytes, s);
+    zero_bytes(Bd.p, Bd.bytes, s);
     rocblas_handle h = blas_handle();
     blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
     auto run = [&](auto tag) {

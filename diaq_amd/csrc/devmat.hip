@@ -131,6 +131,22 @@ void release_cached_memory() {
     release_all_locked(c);
 }
 
+__global__ void zero_bytes_kernel(uint4 *__restrict__ p16, int64_t n16, uint8_t *__restrict__ tail, int ntail) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        p16[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < ntail) tail[threadIdx.x] = 0;
+}
+
+void zero_bytes(void *p, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    if (reinterpret_cast<uintptr_t>(p) % 16 != 0) throw ArgError("zero_bytes: pointer not 16-byte aligned");
+    const int64_t n16 = static_cast<int64_t>(bytes / 16);
+    const int ntail = static_cast<int>(bytes % 16);
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(std::max<unsigned>(1, blocks_for(n16, 256, 8192))), dim3(256), 0, s,
+                       static_cast<uint4 *>(p), n16, static_cast<uint8_t *>(p) + n16 * 16, ntail);
+    DQ_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // Layout
 // ------------------------------------------------------------------------------------------------------------------
@@ -167,8 +183,8 @@ std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32
     M->re = DevBuf(vb);
     M->im = DevBuf(vb);
     if (vb && zero_fill) {
-        DQ_HIP(hipMemsetAsync(M->re.p, 0, vb, stream));
-        DQ_HIP(hipMemsetAsync(M->im.p, 0, vb, stream));
+        zero_bytes(M->re.p, vb, stream);
+        zero_bytes(M->im.p, vb, stream);
     }
     if (n) {
         M->d_meta = DevBuf(sizeof(DiagMeta) * n);
@@ -552,7 +568,7 @@ std::unique_ptr<dq_devmat> prune(const dq_devmat &A, hipStream_t s) {
         DevBuf dp(sizeof(Piece) * pieces.size()), ds(sizeof(int32_t) * seg_of.size()), df(sizeof(int32_t) * A.n());
         DQ_HIP(hipMemcpyAsync(dp.p, pieces.data(), dp.bytes, hipMemcpyHostToDevice, s));
         DQ_HIP(hipMemcpyAsync(ds.p, seg_of.data(), ds.bytes, hipMemcpyHostToDevice, s));
-        DQ_HIP(hipMemsetAsync(df.p, 0, df.bytes, s));
+        zero_bytes(df.p, df.bytes, s);
         const unsigned g = static_cast<unsigned>(pieces.size());
         if (A.precision == DQ_F64)
             hipLaunchKernelGGL(nonzero_kernel<double>, dim3(g), dim3(256), 0, s, dp.as<Piece>(), ds.as<int32_t>(),
@@ -668,25 +684,61 @@ extern "C" int dq_devmat_convert(const dq_devmat *A, int precision, dq_devmat **
     });
 }
 
+// Diagonal k of the device layout (start[k], len[k]) into a double buffer at the host layout's hstart[k]; one
+// workgroup row per diagonal (grid y), FP32 widened.
+template <typename T>
+__global__ void repack_to_host_layout_kernel(int n, const int64_t *__restrict__ dstart, const int64_t *__restrict__ hstart,
+                                             const int64_t *__restrict__ len, const T *__restrict__ re,
+                                             const T *__restrict__ im, double *__restrict__ ore,
+                                             double *__restrict__ oim) {
+    for (int k = blockIdx.y; k < n; k += gridDim.y) {
+        const int64_t L = len[k], s = dstart[k], h = hstart[k];
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {
+            ore[h + i] = static_cast<double>(re[s + i]);
+            oim[h + i] = static_cast<double>(im[s + i]);
+        }
+    }
+}
+
 extern "C" int dq_devmat_download(const dq_devmat *A, const int64_t *starts, double *re, double *im) {
     return guard([&] {
         if (!A) throw ArgError("matrix is NULL");
         if (A->n() && (!starts || !re || !im)) throw ArgError("dq_devmat_download: NULL host array");
-        if (!A->total) return;
-        const size_t eb = elem_bytes(A->precision);
-        std::vector<char> hr(static_cast<size_t>(A->total) * eb), hi(static_cast<size_t>(A->total) * eb);
-        DQ_HIP(hipMemcpy(hr.data(), A->re.p, hr.size(), hipMemcpyDeviceToHost));
-        DQ_HIP(hipMemcpy(hi.data(), A->im.p, hi.size(), hipMemcpyDeviceToHost));
+        if (!A->total || !A->n()) return;
+        // the host layout's extent; its gaps are written as zeros
+        int64_t lo = INT64_MAX, hi = 0, maxlen = 0;
         for (int k = 0; k < A->n(); ++k) {
-            if (A->precision == DQ_F64) {
-                std::memcpy(re + starts[k], hr.data() + A->start[k] * eb, A->len[k] * eb);
-                std::memcpy(im + starts[k], hi.data() + A->start[k] * eb, A->len[k] * eb);
-            } else {
-                const float *fr = reinterpret_cast<const float *>(hr.data()) + A->start[k];
-                const float *fi = reinterpret_cast<const float *>(hi.data()) + A->start[k];
-                for (int64_t t = 0; t < A->len[k]; ++t) { re[starts[k] + t] = fr[t]; im[starts[k] + t] = fi[t]; }
-            }
+            if (starts[k] < 0) throw ArgError("dq_devmat_download: negative start");
+            lo = std::min(lo, starts[k]);
+            hi = std::max(hi, starts[k] + A->len[k]);
+            maxlen = std::max(maxlen, A->len[k]);
         }
+        if (hi <= lo) return;
+        // repack on the device into the host layout (range [lo, hi)), then one copy per array into the caller's
+        // arrays: no host-side staging buffer or per-diagonal copies
+        const int64_t ext = hi - lo;
+        DevBuf br(static_cast<size_t>(ext) * 8), bi(static_cast<size_t>(ext) * 8);
+        zero_bytes(br.p, br.bytes, 0);
+        zero_bytes(bi.p, bi.bytes, 0);
+        std::vector<int64_t> rel(A->n());
+        for (int k = 0; k < A->n(); ++k) rel[k] = starts[k] - lo;
+        DevBuf dst(sizeof(int64_t) * A->n()), hst(sizeof(int64_t) * A->n()), dln(sizeof(int64_t) * A->n());
+        DQ_HIP(hipMemcpy(dst.p, A->start.data(), dst.bytes, hipMemcpyHostToDevice));
+        DQ_HIP(hipMemcpy(hst.p, rel.data(), hst.bytes, hipMemcpyHostToDevice));
+        DQ_HIP(hipMemcpy(dln.p, A->len.data(), dln.bytes, hipMemcpyHostToDevice));
+        const dim3 grid(std::max<unsigned>(1, blocks_for(maxlen, 256, 64)),
+                        static_cast<unsigned>(std::min(A->n(), 65535)));
+        if (A->precision == DQ_F64)
+            hipLaunchKernelGGL(repack_to_host_layout_kernel<double>, grid, dim3(256), 0, 0, A->n(), dst.as<int64_t>(),
+                               hst.as<int64_t>(), dln.as<int64_t>(), A->re.as<double>(), A->im.as<double>(),
+                               br.as<double>(), bi.as<double>());
+        else
+            hipLaunchKernelGGL(repack_to_host_layout_kernel<float>, grid, dim3(256), 0, 0, A->n(), dst.as<int64_t>(),
+                               hst.as<int64_t>(), dln.as<int64_t>(), A->re.as<float>(), A->im.as<float>(),
+                               br.as<double>(), bi.as<double>());
+        DQ_LAUNCH_CHECK();
+        DQ_HIP(hipMemcpy(re + lo, br.p, br.bytes, hipMemcpyDeviceToHost));
+        DQ_HIP(hipMemcpy(im + lo, bi.p, bi.bytes, hipMemcpyDeviceToHost));
     });
 }
 

@@ -174,6 +174,10 @@ std::unique_ptr<dq_devmat> make_devmat(uint32_t rows, uint32_t cols, const int32
 // made with zero_fill = false whose stored elements a kernel writes.
 void zero_gaps(dq_devmat &M, hipStream_t s);
 
+// bytes zero bytes at p (16-byte aligned) by a fill kernel on s: cheaper on the host than the runtime's memset, and
+// safe to capture into a graph.
+void zero_bytes(void *p, size_t bytes, hipStream_t s);
+
 // Launch geometry helpers.
 inline unsigned blocks_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
     int64_t b = (work + per_block - 1) / per_block;

@@ -327,8 +327,8 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
         if (A->rows == 0 || A->n() == 0) return;
         const size_t eb = elem_bytes(A->precision);
         DevBuf xr(A->cols * eb), xi(A->cols * eb), yr(A->rows * eb), yi(A->rows * eb);
-        DQ_HIP(hipMemsetAsync(xr.p, 0, xr.bytes, s));
-        DQ_HIP(hipMemsetAsync(xi.p, 0, xi.bytes, s));
+        zero_bytes(xr.p, xr.bytes, s);
+        zero_bytes(xi.p, xi.bytes, s);
         hipEvent_t e0, e1;
         DQ_HIP(hipEventCreate(&e0));
         DQ_HIP(hipEventCreate(&e1));

@@ -88,8 +88,10 @@ int dq_devmat_copy(const dq_devmat *A, dq_devmat **out);
 /* A copy in another precision (FP64 <-> FP32). No reference counterpart (precision form). */
 int dq_devmat_convert(const dq_devmat *A, int precision, dq_devmat **out);
 
-/* Download into host arrays laid out by starts[] (as dq_devmat_upload); values are widened to double.
- * Replaces deviceDiaQMatrix::to_host (src/deviceMatrix_gpu.hpp:55, .cu:97-113). */
+/* Download into host arrays laid out by starts[] (as dq_devmat_upload); values are widened to double, and the
+ * elements between the first start and the last diagonal's end that belong to no diagonal (the layout's padding)
+ * are written as zeros. Repacked on the device, then one copy per array. Replaces deviceDiaQMatrix::to_host
+ * (src/deviceMatrix_gpu.hpp:55, .cu:97-113). */
 int dq_devmat_download(const dq_devmat *A, const int64_t *starts, double *re, double *im);
 
 /* Download one diagonal (by offset) into host arrays of its length, widened to double. No reference counterpart (the
