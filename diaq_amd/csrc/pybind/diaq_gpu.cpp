@@ -858,6 +858,26 @@ PYBIND11_MODULE(diaq_gpu, m) {
         return product_dev(*A, *B);
     }, py::arg(), py::arg());
     m.def("trace", [](std::shared_ptr<DeviceMatrix> A) { return trace_dev(*A); });
+    // pow / expm / expm_negative on a DeviceMatrix: the result stays on the device (the host-matrix forms above return
+    // a host diaQMatrix, as the reference's do, which for a near-dense result means downloading it)
+    m.def("pow", [](std::shared_ptr<DeviceMatrix> A, long long n) {
+        dq_devmat *p = nullptr;
+        check(dq_pow(A->h.get(), n, &p));
+        return wrap(p);
+    }, py::arg(), py::arg());
+    m.def("expm", [](std::shared_ptr<DeviceMatrix> A, int max_iterations, double scale_cutoff, double tolerance) {
+        dq_devmat *p = nullptr;
+        check(dq_expm_taylor_scaled(A->h.get(), max_iterations, scale_cutoff, tolerance, &p));
+        return wrap(p);
+    }, py::arg("A"), py::arg("max_iterations") = 100, py::arg("scale_cutoff") = 1.0, py::arg("tolerance") = 1e-12);
+    m.def("expm_negative", [](std::shared_ptr<DeviceMatrix> A, double t, int max_iterations, double scale_cutoff,
+                              double tolerance) {
+        dq_devmat *p = nullptr;
+        check(dq_expm_negative_taylor_scaled(A->h.get(), t, max_iterations, scale_cutoff, tolerance, &p, nullptr,
+                                             nullptr));
+        return wrap(p);
+    }, py::arg("A"), py::arg("t"), py::arg("max_iterations") = 100, py::arg("scale_cutoff") = 1.0,
+       py::arg("tolerance") = 1e-12);
     m.def("prune", [](std::shared_ptr<DeviceMatrix> A) {
         dq_devmat *p = nullptr;
         check(dq_prune(A->h.get(), &p));

@@ -1051,3 +1051,19 @@ def test_reference_api_vs_numpy_script(d):
         check("expm_negative", N(d.expm_negative(M(H), 0.3, 100, 1.0, 1e-14)), scipy.linalg.expm(-1j * 0.3 * H),
               1e-8)
     assert len(worst) == 11  # every named check ran
+
+
+def test_pow_expm_on_device_matrices(d):
+    """pow / expm / expm_negative on a DeviceMatrix return a DeviceMatrix with the same offsets and the same bits as
+    the host-matrix forms (the same device computation, without the download)."""
+    import scipy.linalg
+    rows, cols, offs, vals = Hm.tfim(8, 1.0, 0.7)
+    H = dia_of(d, rows, cols, offs, vals)
+    D = d.to_device(H)
+    for host, dev in ((d.expm_negative(H, 0.3), d.expm_negative(D, 0.3)), (d.expm(H), d.expm(D)),
+                      (d.pow(H, 3), d.pow(D, 3))):
+        assert isinstance(dev, type(D))
+        assert list(host.dIndices) == list(dev.diagonals)
+        np.testing.assert_array_equal(d.to_numpy(dev.to_host()), d.to_numpy(host))
+    dense = O.to_dense(O.Dia(rows, cols, offs, vals))
+    assert rel(d.to_numpy(d.expm_negative(D, 0.3).to_host()), scipy.linalg.expm(-1j * 0.3 * dense)) < 1e-12
