@@ -687,10 +687,10 @@ extern "C" int dq_devmat_convert(const dq_devmat *A, int precision, dq_devmat **
 // Diagonal k of the device layout (start[k], len[k]) into a double buffer at the host layout's hstart[k]; one
 // workgroup row per diagonal (grid y), FP32 widened.
 template <typename T>
-__global__ void repack_to_host_layout_kernel(int n, const int64_t *__restrict__ dstart, const int64_t *__restrict__ hstart,
-                                             const int64_t *__restrict__ len, const T *__restrict__ re,
-                                             const T *__restrict__ im, double *__restrict__ ore,
-                                             double *__restrict__ oim) {
+__global__ void repack_to_host_layout_kernel(int n, const int64_t *__restrict__ dstart,
+                                             const int64_t *__restrict__ hstart, const int64_t *__restrict__ len,
+                                             const T *__restrict__ re, const T *__restrict__ im,
+                                             double *__restrict__ ore, double *__restrict__ oim) {
     for (int k = blockIdx.y; k < n; k += gridDim.y) {
         const int64_t L = len[k], s = dstart[k], h = hstart[k];
         for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < L; i += (int64_t)gridDim.x * blockDim.x) {

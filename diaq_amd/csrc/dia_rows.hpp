@@ -98,9 +98,10 @@ __device__ inline void stage_x_window(int64_t tile, int64_t cols, const T *__res
 
 // Zero-block map of a DIA layout: bit b of word [job][diagonal][b / 32] is set when some value of the diagonal in
 // rows [b * BR, (b + 1) * BR) is nonzero (a NaN counts as nonzero). One workgroup per (word, diagonal, job), the
-// diagonals strided over gridDim.y (at most 65535); the values of job j start at j * total. A block whose bit is clear adds nothing to any row sum, so the row-tile loops
-// can skip its value loads and x gathers: a qubit Hamiltonian's +-2^k flip diagonals are zero on alternate runs of
-// 2^k rows (three runs in four for a flip-flop term), half or more of what they store.
+// diagonals strided over gridDim.y (at most 65535); the values of job j start at j * total. A block whose bit is
+// clear adds nothing to any row sum, so the row-tile loops can skip its value loads and x gathers: a qubit
+// Hamiltonian's +-2^k flip diagonals are zero on alternate runs of 2^k rows (three runs in four for a flip-flop
+// term), half or more of what they store.
 template <typename T, int BR>
 __global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t rows, int n, const DiagMeta *__restrict__ meta,
                                                           int64_t total, const T *__restrict__ are,
@@ -134,7 +135,9 @@ __global__ __launch_bounds__(256) void zero_blocks_kernel(int64_t rows, int n, c
 // Grid rows for a zero-block map over n diagonals.
 inline unsigned zero_map_grid_y(int n) { return static_cast<unsigned>(n < 65535 ? n : 65535); }
 
-inline int64_t zero_block_words(int64_t rows, int block_rows) { return ((rows + block_rows - 1) / block_rows + 31) / 32; }
+inline int64_t zero_block_words(int64_t rows, int block_rows) {
+    return ((rows + block_rows - 1) / block_rows + 31) / 32;
+}
 
 // PIPE = true: each diagonal's 4*RPT loads are issued together (full tiles unconditionally, boundary tiles with
 // clamped addresses), the schedule that keeps the most bytes in flight. Every schedule adds exactly the products of

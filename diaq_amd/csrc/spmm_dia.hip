@@ -7,11 +7,12 @@
 // n_right, both row-major; double or interleaved complex double.
 //
 // Default mapping on gfx950 (spmm_dia_cols_kernel): a lane owns two adjacent columns of one row of C when n_right is
-// even, one element otherwise; many rows with wide right-hand sides use the row-block form (spmm_dia_rows_kernel). One element per lane (spmm_dia_kernel): lane e owns C's flat element
-// e = r * n_right + j. Its B operand for diagonal k is
-// B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run of B and
-// writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles rows,
-// which coalesces only for wide B). The diagonal values vals[k * n_rows + r] are shared by the n_right lanes of a row.
+// even, one element otherwise; many rows with wide right-hand sides use the row-block form (spmm_dia_rows_kernel).
+// One element per lane (spmm_dia_kernel): lane e owns C's flat element e = r * n_right + j. Its B operand for
+// diagonal k is B[(r + o_k) * n_right + j] = B[e + o_k * n_right]: for every diagonal a wave reads one contiguous run
+// of B and writes one contiguous run of C, whatever n_right is (the reference assigns a column per thread and tiles
+// rows, which coalesces only for wide B). The diagonal values vals[k * n_rows + r] are shared by the n_right lanes of
+// a row.
 // Each element sums its diagonals in order, as the reference does; the loads of 8 diagonals are issued before their
 // products are added, so a lane keeps 16 loads in flight, and a diagonal's offset and base pointers are uniform
 // (scalar) values.

@@ -36,8 +36,8 @@ static int g_spmv_pipe = -1;  // load schedule: -1 = automatic, 0 = per-row pred
 static int g_spmv_lds = -1;   // LDS x window: -1 = automatic, 0 = off, 1 = on
 static int g_spmv_xcd = -1;   // tile order: -1 = automatic, 0 = round-robin over the XCDs, 1 = XCD-contiguous
 
-// ZM: zm is a zero-block map of A (256-row blocks, zwords words per diagonal) whose all-zero blocks are skipped;
-// only callers that hold A fixed for the map's lifetime pass one (spMV_ntimes).
+// ZM: zm is a zero-block map of A (256-row blocks, zwords words per diagonal) whose all-zero blocks are skipped: the
+// map dq_spmv_tune keeps in the handle (dropped when the values are rewritten), or one spMV_ntimes makes for a call.
 template <typename T, int RPT, bool NT, bool PIPE, bool LDS, bool ZM>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int64_t rows, int64_t cols, int n,
                                                         const DiagMeta *__restrict__ meta, const T *__restrict__ are,
@@ -237,6 +237,13 @@ static const uint32_t *handle_map(const dq_devmat &A) {
     return (g_spmv_skip && A.zmap_valid && A.precision == DQ_F64) ? A.zmap.as<uint32_t>() : nullptr;
 }
 
+// The handle's map for a sequence of n products: a single product follows handle_map; two or more use the map in
+// any precision (their results are not compared with a single untuned SpMV's).
+static const uint32_t *sequence_map(const dq_devmat &A, int n) {
+    if (n == 1) return handle_map(A);
+    return (g_spmv_skip && A.zmap_valid) ? A.zmap.as<uint32_t>() : nullptr;
+}
+
 static size_t ntimes_map_bytes(const dq_devmat &A, int64_t &zwords) {
     zwords = zero_block_words(A.rows, 256);
     return sizeof(uint32_t) * static_cast<size_t>(zwords) * static_cast<size_t>(A.n());
@@ -267,7 +274,6 @@ static void ntimes_zero_map(const dq_devmat &A, int64_t zwords, uint32_t *z, hip
                            A.n(), A.d_meta.as<DiagMeta>(), int64_t(0), A.re.as<float>(), A.im.as<float>(), zwords, z);
     DQ_LAUNCH_CHECK();
 }
-
 
 // y = A x: the schedule of A, skipping its all-zero blocks when the handle holds their map (see handle_map).
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s) {
@@ -517,7 +523,7 @@ extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const v
         // must not read memory this call has given back).
         int64_t zwords = 0;
         uint32_t *zmap = nullptr;
-        const uint32_t *zuse = n == 1 ? handle_map(*A) : ((g_spmv_skip && A->zmap_valid) ? A->zmap.as<uint32_t>() : nullptr);
+        const uint32_t *zuse = sequence_map(*A, n);
         if (zuse) zwords = A->zmap_words;
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         DQ_HIP(hipStreamIsCapturing(s, &cap));
@@ -570,7 +576,7 @@ extern "C" int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const d
         a.put(xr, xi);
         int64_t zwords = 0;
         DevBuf zmap;
-        const uint32_t *zuse = n == 1 ? handle_map(*A) : ((g_spmv_skip && A->zmap_valid) ? A->zmap.as<uint32_t>() : nullptr);
+        const uint32_t *zuse = sequence_map(*A, n);
         if (zuse) zwords = A->zmap_words;
         if (!zuse && ntimes_uses_map(*A, n)) {
             zmap = DevBuf(ntimes_map_bytes(*A, zwords));
