@@ -97,7 +97,6 @@ __global__ void copy2_kernel(int64_t n, const T *__restrict__ ar, const T *__res
     }
 }
 
-// One step out = sum_{k=0}^{m} (-i dt H)^k psi / k! (src/spHamSim_gpu_all_fused.cu:235-277). t0/t1: scratch terms.
 // One term: term_k = (-i scale) H in, out (+)= term_k, on tiles of 256 * RPT rows.
 template <typename T, int RPT>
 static void launch_term(const dq_devmat &H, bool lds, const T *inr, const T *ini, T *nr, T *ni, T *outr, T *outi,
