@@ -97,6 +97,31 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     log(f"[bench] spmv fp64: {ms:.3f} ms/step  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS:.3f} of peak)  wall {wall:.2f}s")
     res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg, tuned=tuned)
 
+    # The same product through the host-vector entry point, for callers that hand over host buffers: x goes up and y
+    # comes back over PCIe on every call (pageable numpy arrays, as the reference's spmv_cuda(DeviceMatrix, x)).
+    # Context only, never the headline; the full bench only, so that profiling runs time the device entry alone.
+    try:
+        if getattr(args, "profile", None) is not None:
+            raise RuntimeError("skipped in profiling runs")
+        hx = [t.cpu().numpy() for t in (xr, xi)]
+        hy = [np.empty(N), np.empty(N)]
+        args_h = [capi.ptr(a, ctypes.c_double) for a in (*hx, *hy)]
+        capi.check(L.dq_spmv_host(out, *args_h))
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            capi.check(L.dq_spmv_host(out, *args_h))
+        ms_h = (time.perf_counter() - t0) / reps * 1e3
+        same = bool(np.array_equal(hy[0], yr.cpu().numpy()) and np.array_equal(hy[1], yi.cpu().numpy()))
+        res["host_vectors"] = dict(ms=ms_h, gbs_alg=bytes_alg / (ms_h * 1e-3) / 1e9, pcie_bytes=4 * 8 * N,
+                                   same_bits_as_device_entry=same, reps=reps)
+        log(f"[bench] spmv fp64 with host vectors (x up, y down over PCIe per call): {ms_h:.2f} ms = "
+            f"{res['host_vectors']['gbs_alg']:.0f} GB/s of algorithmic bytes; same bits {same}")
+        del hx, hy
+    except Exception as e:  # context only
+        res["host_vectors"] = dict(error=str(e))
+        log(f"[bench] spmv host vectors failed: {e}")
+
     # FP32 form of the same matrix (precision form): half the bytes
     try:
         if not fp32:
@@ -1033,6 +1058,7 @@ def main():
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
                          else None),
         "spmv_fp32": sp.get("fp32"),
+        "spmv_host_vectors": sp.get("host_vectors"),
         "spmv_vendor_csr": sp.get("vendor_csr"),
         "cpu_detail": cpu or None,
         "spmv_structured": extra.get("spmv_structured"),
