@@ -32,7 +32,8 @@ enum { DQ_F64 = 0, DQ_F32 = 1 }; /* value precision of a device matrix */
 /* Opaque device-resident DIA matrix: deviceDiaQMatrix (src/deviceMatrix_gpu.hpp:37-68). */
 typedef struct dq_devmat dq_devmat;
 
-/* Message of this thread's last failed call ("" if none). */
+/* Message of this thread's last failed call ("" if none). No reference counterpart: the reference throws C++
+ * exceptions, which cannot cross a C ABI. */
 const char *dq_last_error(void);
 
 /* Library version string. Replaces version() (include/pythonWrappers.cpp:15-17). */
