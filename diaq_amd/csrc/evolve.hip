@@ -336,7 +336,7 @@ struct dq_evo {
     std::vector<dq::EvoClass> classes;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool renorm_in_graph = false;
+    std::vector<int64_t> graph_sig;  // what the built graph's launches depend on (graph_signature)
     ~dq_evo() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -476,6 +476,15 @@ static void run_class_levels(dq_evo &E, EvoClass &c, hipStream_t s, bool renorm)
         for (int64_t g = 0; g < c.G; ++g) launch_level<TV, NS>(c, g, s);
         if (renorm) launch_renorm<NS>(c, s);
     });
+}
+
+// What a captured graph's launches depend on: the renormalisation flag, the process-wide Horner overrides, and per
+// class its number of levels and its tuned schedule. The kernels read each job's (tau, m, s) from d_params when they
+// run, so a new schedule that keeps every class's number of levels replays the same graph.
+static std::vector<int64_t> graph_signature(const dq_evo &E, bool renorm) {
+    std::vector<int64_t> sig{renorm ? 1 : 0, g_evo_rpt, g_evo_xcd, g_evo_lds, g_evo_skip};
+    for (const auto &c : E.classes) sig.insert(sig.end(), {c.G, c.rpt, c.xcd, c.lds});
+    return sig;
 }
 
 }  // namespace dq
@@ -630,7 +639,8 @@ extern "C" int dq_evo_get_states(const dq_evo *E, int job, double *re, double *i
     });
 }
 
-// Per-job schedule: evolve to t[j] with 2^s[j] sub-steps of degree m[j]. Invalidates a built graph.
+// Per-job schedule: evolve to t[j] with 2^s[j] sub-steps of degree m[j]. The parameters go to device memory, where
+// the kernels read them; dq_evo_run rebuilds its graph only when some class's number of levels changed.
 extern "C" int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int32_t *s) {
     return guard([&] {
         if (!E || (E->n_jobs && (!t || !m || !s))) throw ArgError("dq_evo_set_schedule: bad arguments");
@@ -648,8 +658,6 @@ extern "C" int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m,
             }
             DQ_HIP(hipMemcpy(c.d_params.p, c.params.data(), sizeof(JobParam) * c.jobs_n(), hipMemcpyHostToDevice));
         }
-        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
-        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
     });
 }
 
@@ -733,7 +741,8 @@ extern "C" int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t s
             }
         };
         if (use_graph) {
-            if (E->exec && E->renorm_in_graph != (renormalize != 0)) {
+            std::vector<int64_t> sig = graph_signature(*E, renormalize != 0);
+            if (E->exec && E->graph_sig != sig) {
                 (void)hipGraphExecDestroy(E->exec); E->exec = nullptr;
                 (void)hipGraphDestroy(E->graph); E->graph = nullptr;
             }
@@ -753,7 +762,7 @@ extern "C" int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t s
                 DQ_HIP(hipStreamEndCapture(cap, &E->graph));
                 DQ_HIP(hipStreamDestroy(cap));
                 DQ_HIP(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
-                E->renorm_in_graph = renormalize != 0;
+                E->graph_sig = std::move(sig);
             }
             DQ_HIP(hipEventRecord(e0, s0));
             DQ_HIP(hipGraphLaunch(E->exec, s0));

@@ -348,7 +348,10 @@ int dq_evo_probe(dq_evo *E, double *est);
 
 /* Evolve every job by its schedule; use_graph != 0 captures the whole batch (one stream per class) into a HIP graph
  * on the first call and replays it; renormalize != 0 ends with an FP64 renormalisation of every state. ms_out (may
- * be NULL) receives the GPU time. */
+ * be NULL) receives the GPU time. The graph is captured again only when what its launches depend on changed: a
+ * class's number of Horner levels (max over its jobs of m 2^s), the tuned schedule, the evo_* tuning keys or
+ * renormalize. A new schedule that keeps those replays the same graph (the kernels read t, m and s from device
+ * memory). */
 int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t stream, float *ms_out);
 
 /* ------------------------------------------------------------------------------------------------------------------

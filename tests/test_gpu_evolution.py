@@ -61,6 +61,35 @@ def test_batched_vs_dense_expm_and_graph_replays(mods, prec):
             np.testing.assert_array_equal(B.get_states(q, 1 << j["n"]), eager[q])
 
 
+def test_graph_follows_new_schedules(mods):
+    """A new (t, m, s) that keeps every class's number of Horner levels (max over its jobs of m 2^s) replays the
+    captured graph, which must then run the new schedule; one that changes a level count is captured again. Every
+    graph run equals the eager run of the same schedule bit for bit."""
+    d, ev = mods
+    jobs = small_batch(d, seed=3, n_states=2)  # sizes 6, 7, 8, 6, 8, 9: classes {0, 3}, {1}, {2, 4}, {5}
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=2, value_precision="fp32")
+
+    def run(t, ms, graph):
+        B.set_schedule(t, [m for m, s in ms], [s for m, s in ms])
+        for q, j in enumerate(jobs):
+            B.set_states(q, j["states"])
+        B.run(use_graph=graph, renormalize=True)
+        return [B.get_states(q, 1 << j["n"]) for q, j in enumerate(jobs)]
+
+    t0 = [j["t"] for j in jobs]
+    t1 = [0.6 * t for t in t0]
+    levels = [(6, 1), (5, 2), (8, 0), (4, 1), (7, 1), (3, 2)]  # per class: 12, 20, 14, 12 levels
+    same = [(3, 2), (10, 1), (7, 1), (12, 0), (2, 2), (6, 1)]  # other jobs' schedules, the same class maxima
+    more = [(3, 2), (6, 2), (7, 1), (12, 0), (2, 2), (6, 1)]   # class {1}: 24 levels
+    for t, ms in ((t0, levels), (t1, same), (t0, same), (t1, more), (t0, levels)):
+        graph = run(t, ms, True)
+        eager = run(t, ms, False)
+        for g, e in zip(graph, eager):
+            np.testing.assert_array_equal(g, e)
+    # the runs above differ from each other: a stale schedule could not pass
+    assert not np.array_equal(run(t0, levels, True)[0], run(t1, same, True)[0])
+
+
 def test_batched_matches_taylor_action_oracle_for_any_schedule(mods):
     d, ev = mods
     jobs = small_batch(d, seed=1, n_states=2)
