@@ -98,6 +98,30 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
     return r;
 }
 
+// The dry-run depends only on the shape and the two offset lists. A Taylor series multiplies operands with the same
+// offsets term after term once the terms reach every diagonal they can (TFIM-12: terms 8 to 14, 8191 x 25 offsets,
+// ~1.2 ms of host time each), so each thread keeps its last dry-run and reuses it when the inputs are equal.
+static const DryRun &dryrun_reused(uint32_t rows, uint32_t cols, const std::vector<int32_t> &oa,
+                                   const std::vector<int32_t> &ob) {
+    struct Last {
+        bool valid = false;
+        uint32_t rows = 0, cols = 0;
+        std::vector<int32_t> oa, ob;
+        DryRun d;
+    };
+    thread_local Last last;
+    if (!(last.valid && last.rows == rows && last.cols == cols && last.oa == oa && last.ob == ob)) {
+        last.valid = false;
+        last.d = dryrun(rows, cols, oa, ob);
+        last.rows = rows;
+        last.cols = cols;
+        last.oa = oa;
+        last.ob = ob;
+        last.valid = true;
+    }
+    return last.d;
+}
+
 constexpr int PROD_BATCH = 256;
 
 // Process-wide knobs (dq_set_tuning): chunk order across XCDs and non-temporal stores of C.
@@ -251,7 +275,7 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     if (A.cols != B.rows) throw ArgError("diaQMatrix product_cuda: dimension mismatch.");
     if (A.precision != B.precision) throw ArgError("product: operands have different precisions");
     if (allow_dense && dense_product_pays(A, B)) return product_dense(A, B, divisor, s);
-    const DryRun d = dryrun(A.rows, B.cols, A.offs, B.offs);
+    const DryRun &d = dryrun_reused(A.rows, B.cols, A.offs, B.offs);
     // every stored element of C is written by the kernel; only the gaps between diagonals are zeroed
     auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s, false);
     const int nc = C->n();

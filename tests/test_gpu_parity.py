@@ -330,6 +330,23 @@ def test_product_device_structure_and_values(d, shapes, oa, ob):
     assert Pr.diagonals == O.prune(ref).offsets
 
 
+def test_products_in_a_row_with_equal_offsets(d):
+    """Consecutive products whose operands have the same offset lists (the dry-run of the previous call is reused):
+    new values, a new shape and a new inner dimension each give the oracle's structure and values."""
+    rng = np.random.default_rng(12)
+    oa, ob = [-7, -1, 0, 3, 40], [-40, -2, 0, 1, 9]
+    for m, k, n in ((300, 300, 300), (300, 300, 300), (303, 303, 303), (200, 260, 180), (200, 310, 180),
+                    (300, 300, 300)):
+        A = banded(m, oa, rng, cols=k)
+        B = banded(k, ob, rng, cols=n)
+        MA = O.Dia(m, k, oa, [A[O.position_to_rc(o, np.arange(O.diagonal_length(o, m, k)))] for o in oa])
+        MB = O.Dia(k, n, ob, [B[O.position_to_rc(o, np.arange(O.diagonal_length(o, k, n)))] for o in ob])
+        ref = O.product(MA, MB)
+        C = d.product(d.to_device(dia_of(d, m, k, oa, MA.vals)), d.to_device(dia_of(d, k, n, ob, MB.vals)))
+        assert C.diagonals == ref.offsets, (m, k, n)
+        assert rel(d.to_numpy(C.to_host()), O.to_dense(ref)) < TOL, (m, k, n)
+
+
 def test_prune_constructed_cancellation(d):
     A = d.to_device(d.from_numpy(np.array([[1, 1], [0, 1]], complex)))
     B = d.to_device(d.from_numpy(np.array([[1, -1], [0, 1]], complex)))
