@@ -968,8 +968,8 @@ def main():
     ap.add_argument("--no-evolution", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", nargs="?", const="spmv", default=None,
-                    choices=["spmv", "spmv_structured", "spmv_ntimes", "spgemm", "spmm_dia", "evolution", "batched",
-                             "conversion", "trotter", "builders"],
+                    choices=["spmv", "spmv_structured", "spmv_ntimes", "spgemm", "spmm_dia", "expm", "evolution",
+                             "batched", "conversion", "trotter", "builders"],
                     help="profiling run of one kernel's workload (default: the FP64 SpMV steps), then one plain copy "
                          "of 2 GiB (a kernel of known size to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -1002,6 +1002,10 @@ def main():
         elif args.profile == "spmm_dia":
             sm = spmm_section(args, torch, d, capi, Hm)
             log(f"[bench] profile: spmm_dia {sm['ms']:.3f} ms")
+        elif args.profile == "expm":
+            ex = expm_section(args, torch, d, capi, Hm)
+            log(f"[bench] profile: expm {ex['seconds']:.3f} s (host matrices), {ex['seconds_device_matrix']:.3f} s "
+                "(DeviceMatrix)")
         elif args.profile == "conversion":
             cv = conversion_section(args, torch, d, capi, Hm)
             log(f"[bench] profile: from_dense {cv['from_dense_ms']:.3f} ms, to_dense {cv['to_dense_ms']:.3f} ms")
