@@ -2,7 +2,7 @@
 
 ``diaq_amd.diaq_gpu`` is the drop-in for the reference's ``diaq_gpu`` module (importing this package also makes
 ``import diaq_gpu`` work); ``diaq_amd.capi`` is the ctypes view of the C ABI (include/diaq_hip.h). Both need the
-native libraries built in-tree (``python -m diaq_amd.build``): there is no CPU fallback, a missing library is an
+native libraries built in-tree (``python diaq_amd/build.py``): there is no CPU fallback, a missing library is an
 ImportError.
 """
 import os
@@ -21,7 +21,7 @@ except ImportError:  # torch is optional: without it the system HIP runtime is u
 try:
     from . import diaq_gpu  # noqa: F401  (links libdiaq_hip.so next to it)
 except ImportError as e:  # pragma: no cover - depends on the build
-    raise ImportError(f"diaq_amd: native module not built or not loadable ({e}); run python -m diaq_amd.build") from e
+    raise ImportError(f"diaq_amd: native module not built or not loadable ({e}); run python diaq_amd/build.py") from e
 
 sys.modules.setdefault("diaq_gpu", diaq_gpu)
 

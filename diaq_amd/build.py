@@ -1,6 +1,6 @@
 """Build the native parts in-tree: libdiaq_hip.so (HIP kernels + C ABI, gfx950) and the diaq_gpu pybind11 module.
 
-    python -m diaq_amd.build          # or __graft_entry__.build()
+    python diaq_amd/build.py          # or __graft_entry__.build()
 
 Objects go to diaq_amd/_build/ (git-ignored); the shared libraries land next to this file so they travel with the
 source tree.

@@ -113,7 +113,7 @@ def lib():
         except ImportError:
             pass
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is not built (python -m diaq_amd.build)")
+            raise ImportError(f"{LIB_PATH} is not built (python diaq_amd/build.py)")
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
