@@ -601,4 +601,6 @@ PINS = {
                   "the spKron.cpp:2231-2262 prime-gate case against the kron matrix",
     "dia_viability": "hand-computed budget cases",
     "mlp_loss_grad": "central finite differences of mlp_forward's loss",
+    "adam_step": "closed forms worked by hand: step 1 from zero moments, step 2 with a zero gradient, a constant "
+                 "gradient (tests/test_oracle.py::test_adam_step_closed_forms)",
 }

@@ -437,6 +437,31 @@ def test_mlp_gradient_vs_finite_differences():
     np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
 
 
+def test_adam_step_closed_forms():
+    """Adam from zero moments (b1 = 0.9, b2 = 0.999, eps = 1e-8, lr = 0.1), worked by hand.
+
+    Step 1: m = (1-b1) g and v = (1-b2) g^2, so the bias-corrected moments are g and g^2 and the update is
+    -lr g / (|g| + eps) = -0.0999999995 for g = 2. Step 2 with g = 0: m = b1 (1-b1) g1 and v = b2 (1-b2) g1^2, corrected
+    by 1 - b1^2 = (1-b1)(1+b1) and 1 - b2^2, giving b1 g1 / (1+b1) and b2 g1^2 / (1+b2); the update is
+    -lr (0.9/1.9) 2 / (sqrt(0.999/1.999) 2 + eps) = -0.0670058249397... A constant gradient keeps both corrected
+    moments at g and g^2, so every step moves by lr g / (|g| + eps). Dropping the bias correction would give
+    lr (0.1 g) / (sqrt(0.001) |g|) = 0.316 at step 1; correcting with step - 1 divides by zero."""
+    p = np.array([1.0, -3.0, 0.5])
+    g1 = np.array([2.0, -2.0, 0.0])
+    m = np.zeros(3)
+    v = np.zeros(3)
+    O.adam_step(p, g1, m, v, 1, 0.1)
+    np.testing.assert_allclose(p, [1.0 - 0.0999999995, -3.0 + 0.0999999995, 0.5], rtol=0, atol=1e-15)
+    O.adam_step(p, np.zeros(3), m, v, 2, 0.1)
+    np.testing.assert_allclose(p, [1.0 - 0.0999999995 - 0.06700582493973309,
+                                   -3.0 + 0.0999999995 + 0.06700582493973309, 0.5], rtol=0, atol=1e-15)
+    q = np.array([0.0])
+    mq, vq = np.zeros(1), np.zeros(1)
+    for step in range(1, 6):
+        O.adam_step(q, np.array([-4.0]), mq, vq, step, 0.01)
+        assert q[0] == pytest.approx(step * 0.01 * 4.0 / (4.0 + 1e-8), rel=1e-12)
+
+
 def test_kron_apply_vs_numpy_kron():
     """The reference's own check (python_tests/test_api_vs_numpy.py:61-80): random square gates of size 2..11,
     before/after in 0..2, against np.kron(np.kron(I, G), I) @ x; y accumulates."""
