@@ -78,6 +78,7 @@ SIGNATURES = {
     "dq_hamsim_host": (c_int, [c_void_p, P(c_double), P(c_double), c_double, c_uint64, c_int, P(c_double),
                                P(c_double)]),
     "dq_hamsim_device": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_uint64, c_int, c_uintptr]),
+    "dq_hamsim_device_f64state": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_uint64, c_int, c_uintptr]),
     "dq_spmm_dia_device": (c_int, [c_uintptr, c_uintptr, c_int, c_int64, c_int64, c_uintptr, c_int64, c_uintptr, c_int,
                                    c_uintptr]),
     "dq_dia_viability": (c_int, [c_int64, c_int64, c_int64, c_int64, c_int64, c_int, c_int, P(DiaViability)]),

@@ -287,6 +287,14 @@ std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s) {
     return C;
 }
 
+std::unique_ptr<dq_devmat> convert_devmat(const dq_devmat &A, int precision, hipStream_t s) {
+    auto C = make_devmat(A.rows, A.cols, A.offs.data(), A.n(), precision, s);
+    std::vector<Seg> segs;
+    for (int k = 0; k < A.n(); ++k) segs.push_back(Seg{C->start[k], A.start[k], -1, A.len[k]});
+    run_segments(segs, *C, &A, nullptr, 1.0, s);
+    return C;
+}
+
 std::unique_ptr<dq_devmat> union_add(const dq_devmat &A, const dq_devmat &B, hipStream_t s) {
     if (A.rows != B.rows || A.cols != B.cols) throw ArgError("diaQMatrix::add: shape mismatch");
     std::vector<int32_t> u(A.offs);
@@ -676,11 +684,8 @@ extern "C" int dq_devmat_copy(const dq_devmat *A, dq_devmat **out) {
 extern "C" int dq_devmat_convert(const dq_devmat *A, int precision, dq_devmat **out) {
     return guard([&] {
         if (!A || !out) throw ArgError("NULL argument");
-        auto C = make_devmat(A->rows, A->cols, A->offs.data(), A->n(), precision, 0);
-        std::vector<Seg> segs;
-        for (int k = 0; k < A->n(); ++k) segs.push_back(Seg{C->start[k], A->start[k], -1, A->len[k]});
-        run_segments(segs, *C, A, nullptr, 1.0, 0);
-        *out = C.release();
+        if (precision != DQ_F64 && precision != DQ_F32) throw ArgError("precision must be DQ_F64 or DQ_F32");
+        *out = convert_devmat(*A, precision, 0).release();
     });
 }
 

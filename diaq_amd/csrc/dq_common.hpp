@@ -196,6 +196,8 @@ double frobenius(const dq_devmat &A, hipStream_t s);
 double estimate_1norm(const dq_devmat &A, hipStream_t s);
 std::unique_ptr<dq_devmat> prune(const dq_devmat &A, hipStream_t s);
 std::unique_ptr<dq_devmat> copy_devmat(const dq_devmat &A, hipStream_t s);
+// The same matrix (shape, offsets, layout) with its values rounded or widened to `precision`.
+std::unique_ptr<dq_devmat> convert_devmat(const dq_devmat &A, int precision, hipStream_t s);
 
 // Product tuning keys ("product_interleave", "product_nt_store"); false if the key is not one of them.
 bool product_set_tuning(const std::string &key, int value);

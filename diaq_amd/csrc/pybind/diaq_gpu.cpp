@@ -933,6 +933,13 @@ PYBIND11_MODULE(diaq_gpu, m) {
         check(dq_hamsim_device(H->h.get(), (void *)pr, (void *)pi, t, r, m, stream));
     }, py::arg("H"), py::arg("psi_real"), py::arg("psi_imag"), py::arg("t"), py::arg("r"), py::arg("m"),
        py::arg("stream") = 0, "the fused evolution in place on device vectors (no reference counterpart)");
+    m.def("hamsim_device_fp64_state", [](std::shared_ptr<DeviceMatrix> H, uintptr_t pr, uintptr_t pi, double t,
+                                         uint64_t r, int m, uintptr_t stream) {
+        check(dq_hamsim_device_f64state(H->h.get(), (double *)pr, (double *)pi, t, r, m, stream));
+    }, py::arg("H"), py::arg("psi_real"), py::arg("psi_imag"), py::arg("t"), py::arg("r"), py::arg("m"),
+       py::arg("stream") = 0,
+       "the fused evolution in place on FP64 device vectors for either precision of H; an fp32 H runs its Taylor "
+       "terms in FP32, sums each step in FP64 and renormalises in FP64 (no reference counterpart)");
 
     m.def("install_info", []() {
         py::dict info;

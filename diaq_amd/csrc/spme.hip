@@ -5,7 +5,8 @@
 // (estimate_1norm, copy_and_scale_by_imag, divide_by_int, add, norm: src/diaQMatrix.hpp). The reference uploads both
 // operands and downloads the product for every Taylor term and runs the helpers on the host; here every matrix stays
 // in HBM: the 1/i of divide_by_int is applied in the product's epilogue, the union-add and the Frobenius norm are
-// device kernels, and only the norm (one double per term) comes back to decide the stop.
+// device kernels, and only the norm (one double per term) comes back to decide the stop. For an FP32 matrix the
+// 1-norm is summed in FP64, the Taylor terms run in FP32 (compensated row sums) and the squarings in FP64.
 #include <cmath>
 #include <cstdio>
 
@@ -79,6 +80,19 @@ static std::unique_ptr<dq_devmat> taylor(const dq_devmat &X, int max_iterations,
     return result;
 }
 
+// U^(2^sc) (src/spME_gpu.cu:282, :67 -> pow_cuda). An FP32 U -- Taylor terms in FP32 -- is squared in FP64: each
+// squaring doubles the relative error it inherits, so FP32 rounding added at every one of sc squarings would grow as
+// 2^sc. The result is rounded back to FP32 once, at the end.
+static std::unique_ptr<dq_devmat> square_phase(std::unique_ptr<dq_devmat> U, int sc, hipStream_t s) {
+    if (sc == 0) return U;  // pow_cuda(U, 1) is a copy of U
+    if (U->precision == DQ_F64) return matrix_pow(*U, int64_t(1) << sc, s);
+    auto U64 = convert_devmat(*U, DQ_F64, s);
+    U.reset();
+    auto P = matrix_pow(*U64, int64_t(1) << sc, s);
+    U64.reset();
+    return convert_devmat(*P, DQ_F32, s);
+}
+
 static int scaling_exponent(double scaled_norm, double cutoff) {
     return (scaled_norm > cutoff) ? static_cast<int>(std::ceil(std::log2(scaled_norm / cutoff))) : 0;
 }
@@ -103,7 +117,7 @@ std::unique_ptr<dq_devmat> expm_negative(const dq_devmat &A, double t, int max_i
     if (-1.0 * scaled_t != 0.0) times_i_scale_copy(A, *X, -1.0 * scaled_t, s);
     auto U = taylor(*X, max_iterations, tol_scaled, "expm_negative_taylor_cuda", terms, s);
     if (s_out) *s_out = sc;
-    return matrix_pow(*U, int64_t(1) << sc, s);
+    return square_phase(std::move(U), sc, s);
 }
 
 std::unique_ptr<dq_devmat> expm(const dq_devmat &A, int max_iterations, double cutoff, double tolerance,
@@ -120,7 +134,7 @@ std::unique_ptr<dq_devmat> expm(const dq_devmat &A, int max_iterations, double c
     scale_real(*As, inv_scale, s);
     const double tol_scaled = std::max(1e-15, tolerance * inv_scale);
     auto U = taylor(*As, max_iterations, tol_scaled, "expm_taylor_cuda", nullptr, s);
-    return matrix_pow(*U, int64_t(1) << sc, s);
+    return square_phase(std::move(U), sc, s);
 }
 
 }  // namespace dq

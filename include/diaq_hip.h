@@ -280,14 +280,23 @@ int dq_expm_negative_taylor_scaled(const dq_devmat *A, double t, int max_iterati
  * ---------------------------------------------------------------------------------------------------------------- */
 
 /* psi(t) = prod_r exp(-i H t/r) psi0 with a fixed-m Taylor action per step (m = max_iterations); host vectors in and
- * out. H must be square FP64. Replaces hamiltonian_simulation_cuda (src/spHamSim_gpu_all_fused.cu:287-376). */
+ * out (FP64). H must be square. Replaces hamiltonian_simulation_cuda (src/spHamSim_gpu_all_fused.cu:287-376). An FP32
+ * H is the precision form: Taylor terms in FP32 (compensated row sums), each step's output summed in FP64, and the
+ * final state renormalised in FP64 to psi0's norm. */
 int dq_hamsim_host(const dq_devmat *H, const double *psi_r, const double *psi_i, double t, uint64_t r,
                    int max_iterations, double *out_r, double *out_i);
 
-/* The same evolution in place on device vectors of H's precision, asynchronous on `stream`. No reference
- * counterpart (the resident form of the fused evolution). */
+/* The same evolution in place on device vectors of H's precision, asynchronous on `stream`. For an FP32 H the state
+ * is widened to FP64 for the evolution (as above) and rounded to FP32 once at the end. No reference counterpart (the
+ * resident form of the fused evolution). */
 int dq_hamsim_device(const dq_devmat *H, void *psi_r_dev, void *psi_i_dev, double t, uint64_t r, int m,
                      uintptr_t stream);
+
+/* The resident evolution with an FP64 state whatever H's precision: for an FP64 H the same as dq_hamsim_device; for
+ * an FP32 H the precision form of dq_hamsim_host, in place on the device state. No reference counterpart (the
+ * precision form's entry point). */
+int dq_hamsim_device_f64state(const dq_devmat *H, double *psi_r_dev, double *psi_i_dev, double t, uint64_t r, int m,
+                              uintptr_t stream);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * DIA x dense on device pointers
