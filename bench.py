@@ -12,6 +12,9 @@ fused path; and the batched evolution of 256 TFIM Hamiltonians x 8 states -- evo
 with learned (m, s) against the plain FP64 path, with state errors against an FP64 reference.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-evolution] [--no-batched]
+
+Two measurement tools live here as well: `python bench.py viability-sweep` (the DIA-vs-CSR sweep behind the MI355X
+viability budgets) and `python bench.py rocprof-summary` (the summaries under profiles/ from rocprofv3's output).
 """
 from __future__ import annotations
 
@@ -83,19 +86,24 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # an event between consecutive launches on the launch stream: the K steps' total and each launch's own time
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
+    ev[0].record(stream)
+    for k in range(args.steps):
         step()
-    e1.record(stream)
+        ev[k + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = ev[0].elapsed_time(ev[-1]) / args.steps
+    per_launch = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
+    ms_median = float(np.median(per_launch))
     bytes_alg = 16 * S + 16 * N + 16 * N
     gbs = bytes_alg / (ms * 1e-3) / 1e9
-    log(f"[bench] spmv fp64: {ms:.3f} ms/step  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS:.3f} of peak)  wall {wall:.2f}s")
-    res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg, tuned=tuned)
+    log(f"[bench] spmv fp64: {ms:.3f} ms/step (median launch {ms_median:.3f} ms)  {gbs:.1f} GB/s  "
+        f"({gbs / HBM_PEAK_GBS:.3f} of peak)  wall {wall:.2f}s")
+    res = dict(N=N, S=S, offsets=offs, ms=ms, gbs=gbs, bytes=bytes_alg, tuned=tuned, ms_median=ms_median,
+               ms_min=float(min(per_launch)), ms_max=float(max(per_launch)))
 
     # The same product through the host-vector entry point, for callers that hand over host buffers: x goes up and y
     # comes back over PCIe on every call (pageable numpy arrays, as the reference's spmv_cuda(DeviceMatrix, x)).
@@ -134,6 +142,7 @@ def spmv_section(args, torch, d, capi, Hm, fp32=True):
         for _ in range(args.warmup):
             capi.check(L.dq_spmv_device(out32, xr32.data_ptr(), xi32.data_ptr(), yr32.data_ptr(), yi32.data_ptr(), sid))
         torch.cuda.synchronize()
+        e0, e1 = ev[0], ev[1]
         e0.record(stream)
         for _ in range(args.steps):
             capi.check(L.dq_spmv_device(out32, xr32.data_ptr(), xi32.data_ptr(), yr32.data_ptr(), yi32.data_ptr(), sid))
@@ -292,7 +301,8 @@ def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True)
         if label == "plain_fp64":
             ref_state = torch.complex(pr, pi)
         else:
-            out["plain_vs_m100_l2"] = float(torch.linalg.vector_norm(torch.complex(pr, pi) - ref_state))
+            oracle_state = torch.complex(pr, pi)  # m = 100: truncation far below FP64 rounding
+            out["plain_vs_m100_l2"] = float(torch.linalg.vector_norm(oracle_state - ref_state))
     S = H.stored
     out["bytes_per_term_all_stored"] = 16 * S + 64 * rows
     # the term kernel skips the 256-row blocks of a diagonal whose values are all zero: they count for no bytes
@@ -307,23 +317,28 @@ def evolution_section(args, torch, d, capi, Hm, steps=100, fp32=True, m100=True)
             log(f"[bench] evolution cpu baseline failed: {e}")
     if not fp32:
         return out
-    # precision form: the same evolution with FP32 values and vectors (Kahan-compensated row sums)
+    # precision form: FP32 values and Taylor terms (Kahan-compensated row sums), each step summed into the FP64 state,
+    # renormalised in FP64 at the end
     H32 = H.astype("fp32")
-    pr = torch.as_tensor(psi0.real, device="cuda").float().contiguous()
-    pi = torch.as_tensor(psi0.imag, device="cuda").float().contiguous()
+    pr = torch.as_tensor(psi0.real, device="cuda").contiguous()
+    pi = torch.as_tensor(psi0.imag, device="cuda").contiguous()
     scratch = [v.clone() for v in (pr, pi)]
-    d.hamsim_device(H32, scratch[0].data_ptr(), scratch[1].data_ptr(), dt, 1, m_plain, sid)  # schedule, untimed
+    d.hamsim_device_fp64_state(H32, scratch[0].data_ptr(), scratch[1].data_ptr(), dt, 1, m_plain, sid)  # schedule
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    d.hamsim_device(H32, pr.data_ptr(), pi.data_ptr(), steps * dt, steps, m_plain, sid)
+    d.hamsim_device_fp64_state(H32, pr.data_ptr(), pi.data_ptr(), steps * dt, steps, m_plain, sid)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    st = torch.complex(pr.double(), pi.double())
+    st = torch.complex(pr, pi)
     b32 = 8 * live_stored(rows, cols, offs, vals, 256) + 32 * rows
     out["fp32"] = dict(m=m_plain, seconds=t, evolutions_per_s=1.0 / t, gbs=b32 * m_plain * steps / t / 1e9,
                        l2_err_vs_plain_fp64=float(torch.linalg.vector_norm(st - ref_state)),
-                       speedup_vs_plain_fp64=out["plain_fp64"]["seconds"] / t)
-    log(f"[bench] evolution fp32: m={m_plain} {t:.3f} s/evolution, error vs FP64 {out['fp32']['l2_err_vs_plain_fp64']:.1e}")
+                       speedup_vs_plain_fp64=out["plain_fp64"]["seconds"] / t,
+                       state="FP64 (terms in FP32, each step summed in FP64, FP64 renormalisation)")
+    if m100:
+        out["fp32"]["l2_err_vs_fp64_oracle_m100"] = float(torch.linalg.vector_norm(st - oracle_state))
+    log(f"[bench] evolution fp32: m={m_plain} {t:.3f} s/evolution, error vs FP64 {out['fp32']['l2_err_vs_plain_fp64']:.1e}"
+        + (f", vs the FP64 m=100 oracle {out['fp32']['l2_err_vs_fp64_oracle_m100']:.1e}" if m100 else ""))
     return out
 
 
@@ -959,6 +974,237 @@ def measured_traffic():
         return None
 
 
+# ----------------------------------------------------------------------------------------------------------------------
+# viability-sweep: the reference's 360-case DIA x dense vs CSR x dense sweep (src/dia_viability.hpp:5-18, 36-63) on
+# this GPU -- spmm_dia_device against the vendor CSR x dense product (torch's sparse CSR matmul: hipSPARSE / rocSPARSE)
+# -- and the fit of the rule's five constants with no case passing the budgets slower in DIA, as many DIA wins kept as
+# possible. The library's dq_dia_viability_mi355x applies the fitted constants.
+#
+#     python -u bench.py viability-sweep [--quick] [--json OUT]
+# ----------------------------------------------------------------------------------------------------------------------
+import itertools  # noqa: E402
+
+VIA_ROWS = (4096, 65536)
+VIA_DIAGS = (1, 4, 16, 64, 256, 1024)
+VIA_LAYOUTS = ("band", "scattered")
+VIA_FILLS = (0.1, 0.25, 0.5, 0.75, 1.0)
+VIA_RIGHT = (1, 16, 128)
+
+# the reference's constants (src/dia_viability.hpp:36-40)
+VIA_REFERENCE = dict(max_bytes_over_csr=0.9, min_budget=16, rows_per_diagonal=32, wide_right=32, wide_budget=8)
+
+
+def via_budgets(n_rows, n_diags, nnz, n_right, p, itemsize=16, index_bytes=8):
+    """The reference's two budgets with constants p (src/dia_viability.hpp:42-63), vectorised over cases."""
+    stored = n_diags.astype(float) * n_rows
+    dia_bytes = stored * itemsize + n_diags * 8.0
+    csr_bytes = nnz.astype(float) * (itemsize + index_bytes) + (n_rows + 1.0) * index_bytes
+    storage_ok = (n_diags > 0) & (dia_bytes <= p["max_bytes_over_csr"] * csr_bytes)
+    budget = np.maximum(p["min_budget"], n_rows // p["rows_per_diagonal"])
+    budget = np.where(n_right >= p["wide_right"], np.minimum(budget, p["wide_budget"]), budget)
+    return storage_ok & (n_diags <= budget)
+
+
+def via_fit(cases):
+    """The constants with no false 'viable' (a case that passes but runs slower in DIA) and the most DIA wins;
+    ties go to the rule nearest the reference's."""
+    n_rows = np.array([c["n_rows"] for c in cases])
+    n_diags = np.array([c["n_diags"] for c in cases])
+    nnz = np.array([c["nnz"] for c in cases])
+    n_right = np.array([c["n_right"] for c in cases])
+    faster = np.array([c["dia_ms"] < c["csr_ms"] for c in cases])
+    best = None
+    grid = itertools.product((0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.25, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0),
+                             (4, 8, 16, 32, 64, 128, 256, 1024), (1, 2, 4, 8, 16, 32, 64, 128),
+                             (16, 32, 64, 128, 1 << 30), (4, 8, 16, 32, 64, 128, 256, 1024))
+    for ratio, mb, rpd, wr, wb in grid:
+        p = dict(max_bytes_over_csr=ratio, min_budget=mb, rows_per_diagonal=rpd, wide_right=wr, wide_budget=wb)
+        v = via_budgets(n_rows, n_diags, nnz, n_right, p)
+        fp = int(np.sum(v & ~faster))
+        tp = int(np.sum(v & faster))
+        dist = (abs(np.log(ratio / 0.9)) + abs(np.log2(mb / 16)) + abs(np.log2(rpd / 32)) +
+                abs(np.log2(min(wr, 1 << 10) / 32)) + abs(np.log2(wb / 8)))
+        key = (fp == 0, tp, -dist)
+        if best is None or key > best[0]:
+            best = (key, p, fp, tp)
+    return best[1], best[2], best[3]
+
+
+def via_score(cases, p):
+    n_rows = np.array([c["n_rows"] for c in cases])
+    n_diags = np.array([c["n_diags"] for c in cases])
+    nnz = np.array([c["nnz"] for c in cases])
+    n_right = np.array([c["n_right"] for c in cases])
+    faster = np.array([c["dia_ms"] < c["csr_ms"] for c in cases])
+    v = via_budgets(n_rows, n_diags, nnz, n_right, p)
+    return dict(viable=int(v.sum()), false_viable=int(np.sum(v & ~faster)), dia_wins_kept=int(np.sum(v & faster)),
+                dia_wins_total=int(faster.sum()))
+
+
+def via_run_case(torch, d, n_rows, n_diags, layout, fill, n_right, rng, reps=5):
+    n_cols = n_rows
+    if layout == "band":
+        offs = np.arange(n_diags) - n_diags // 2
+    else:
+        offs = np.sort(rng.choice(np.arange(-(n_rows - 1), n_cols), size=n_diags, replace=False))
+    dev = "cuda"
+    o = torch.tensor(offs, dtype=torch.int64, device=dev)
+    r = torch.arange(n_rows, device=dev)
+    col = r[None, :] + o[:, None]
+    inside = (col >= 0) & (col < n_cols)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(rng.integers(1 << 30)))
+    keep = inside & (torch.rand(n_diags, n_rows, device=dev, generator=g) < fill)
+    vals = torch.randn(n_diags, n_rows, dtype=torch.complex128, device=dev, generator=g) * keep
+    nnz = int(keep.sum().item())
+    B = torch.randn(n_cols, n_right, dtype=torch.complex128, device=dev, generator=g)
+    C = torch.empty(n_rows, n_right, dtype=torch.complex128, device=dev)
+    s = torch.cuda.current_stream()
+
+    def dia():
+        d.spmm_dia_device(o.data_ptr(), vals.data_ptr(), n_diags, n_rows, n_cols, B.data_ptr(), n_right,
+                          C.data_ptr(), True, s.cuda_stream)
+
+    kk, rr = torch.nonzero(keep, as_tuple=True)
+    cc = rr + o[kk]
+    order = torch.argsort(rr * n_cols + cc)
+    rows_s, cols_s, v_s = rr[order], cc[order], vals[kk, rr][order]
+    crow = torch.zeros(n_rows + 1, dtype=torch.int64, device=dev)
+    crow[1:] = torch.cumsum(torch.bincount(rows_s, minlength=n_rows), 0)
+    A = torch.sparse_csr_tensor(crow, cols_s, v_s, size=(n_rows, n_cols))
+
+    def csr():
+        return A @ B
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_dia = timed(dia)
+    t_csr = timed(csr)
+    err = float((C - csr()).abs().max() / max(1e-300, float(C.abs().max())))
+    return dict(n_rows=n_rows, n_diags=n_diags, layout=layout, fill=fill, n_right=n_right, nnz=nnz,
+                dia_ms=t_dia, csr_ms=t_csr, max_rel_diff=err)
+
+
+def viability_sweep_main(argv):
+    ap = argparse.ArgumentParser(prog="bench.py viability-sweep")
+    ap.add_argument("--quick", action="store_true", help="4096 rows only, 1 and 16 right-hand columns")
+    ap.add_argument("--json", default=None)
+    args = ap.parse_args(argv)
+    import torch
+    import diaq_amd  # noqa: F401
+    import diaq_gpu as d
+    if not torch.cuda.is_available():
+        sys.exit("bench.py viability-sweep needs a GPU")
+    rng = np.random.default_rng(0)
+    rows = VIA_ROWS[:1] if args.quick else VIA_ROWS
+    right = VIA_RIGHT[:2] if args.quick else VIA_RIGHT
+    cases = []
+    t0 = time.time()
+    for n_rows, nd, layout, fill, nr in itertools.product(rows, VIA_DIAGS, VIA_LAYOUTS, VIA_FILLS, right):
+        c = via_run_case(torch, d, n_rows, nd, layout, fill, nr, rng)
+        cases.append(c)
+        print(f"[sweep] rows {n_rows:6d} diags {nd:5d} {layout:9s} fill {fill:4.2f} right {nr:4d}: "
+              f"DIA {c['dia_ms']:.3f} ms, CSR {c['csr_ms']:.3f} ms ({c['csr_ms'] / c['dia_ms']:.2f}x), "
+              f"diff {c['max_rel_diff']:.1e}  [{time.time() - t0:.0f} s]", flush=True)
+    p, fp, tp = via_fit(cases)
+    out = dict(cases=cases, reference_rule=dict(constants=VIA_REFERENCE, **via_score(cases, VIA_REFERENCE)),
+               fitted_rule=dict(constants=p, **via_score(cases, p)))
+    print(json.dumps({k: v for k, v in out.items() if k != "cases"}, indent=1))
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+
+# ----------------------------------------------------------------------------------------------------------------------
+# rocprof-summary: rocprofv3 results (the rocpd SQLite files) as short text -- per-kernel dispatch count and average
+# duration, and per-dispatch FETCH_SIZE / WRITE_SIZE in bytes, each also scaled by what the same counter reads for a
+# plain copy kernel of known size (the calibration dispatch every `bench.py --profile` run ends with).
+#
+#     python bench.py rocprof-summary --stats DB [--fetch DB] [--write DB] [--match spmv_rows_kernel] [--json OUT]
+# ----------------------------------------------------------------------------------------------------------------------
+import sqlite3  # noqa: E402
+import statistics  # noqa: E402
+
+PROF_COPY_BYTES = 1 << 31  # bench.py --profile: dst.copy_(src) of 2^28 doubles
+
+
+def prof_kernel_stats(db, match=None):
+    c = sqlite3.connect(db)
+    out = []
+    for name, calls, total, avg, pct in c.execute("select name, total_calls, total_duration, average, percentage "
+                                                  "from top_kernels"):
+        if match and match not in name:
+            continue
+        out.append((name, calls, total, avg, pct))
+    return out
+
+
+def prof_counter(db, cname, match):
+    """Per-dispatch values (bytes) of one counter for the kernel matching `match` -- when several kernel names match
+    (an autotuner's candidates), the one dispatched most often, i.e. the schedule the timed run used -- and the
+    value the counter reads for the calibration copy."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select kernel_name, value, duration from counters_collection where counter_name=? order by id",
+                     (cname,)).fetchall()
+    by_name = {}
+    for k, v, d in rows:
+        if match in k:
+            by_name.setdefault(k, []).append((v * 1024.0, d))
+    name = max(by_name, key=lambda k: len(by_name[k])) if by_name else None
+    hit = by_name.get(name, [])
+    # calibration: the largest copy-like dispatch (the 2 GiB copy that the profiling run ends with)
+    cal = [v * 1024.0 for k, v, d in rows if ("copyBuffer" in k or "elementwise" in k) and v * 1024.0 > 1e9]
+    return hit, (max(cal) if cal else None), name
+
+
+def rocprof_summary_main(argv):
+    ap = argparse.ArgumentParser(prog="bench.py rocprof-summary")
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--match", default="spmv_rows_kernel")
+    ap.add_argument("--top", type=int, default=6)
+    ap.add_argument("--json", help="also write {kernel, fetch, write, factors, traffic_bytes} to this file")
+    a = ap.parse_args(argv)
+    summary = {}
+    print("kernel stats (rocprofv3 --kernel-trace --stats), durations in microseconds")
+    for name, calls, total, avg, pct in prof_kernel_stats(a.stats)[: a.top]:
+        print(f"  {avg:10.1f} us avg  {calls:4d} calls  {pct:5.1f}%  {name[:110]}")
+    for label, db, cname in (("FETCH_SIZE", a.fetch, "FETCH_SIZE"), ("WRITE_SIZE", a.write, "WRITE_SIZE")):
+        if not db:
+            continue
+        hit, cal, name = prof_counter(db, cname, a.match)
+        if not hit:
+            print(f"{label}: no dispatch matching {a.match}")
+            continue
+        med = statistics.median(v for v, _ in hit)
+        short = name.split("(")[0].replace("void ", "")
+        line = f"{label} per dispatch of {short}: median {med / 1e9:.3f} GB over {len(hit)} dispatches"
+        factor = PROF_COPY_BYTES / cal if cal else None
+        if cal:
+            line += (f"; the 2 GiB calibration copy reads {cal / 1e9:.3f} GB, factor {factor:.3f} -> "
+                     f"{med * factor / 1e9:.3f} GB")
+        print(line)
+        summary[label] = dict(raw_bytes=med, factor=factor, bytes=med * (factor or 1.0), kernel=short)
+    if a.json and "FETCH_SIZE" in summary and "WRITE_SIZE" in summary:
+        import json
+        summary["kernel"] = summary["FETCH_SIZE"]["kernel"]
+        summary["traffic_bytes"] = summary["FETCH_SIZE"]["bytes"] + summary["WRITE_SIZE"]["bytes"]
+        with open(a.json, "w") as f:
+            json.dump(summary, f, indent=1)
+
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1071,7 +1317,12 @@ def main():
                    "N": sp["N"], "n_diags": 64, "stored_elements": sp["S"], "compulsory_bytes": sp["bytes"],
                    "spmv_schedule": sp.get("tuned")},
         "roofline": {"bound": "hbm", "achieved": round(sp["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
+                     "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel and workload, calibrated by "
+                                       "a 2 GiB copy (profiles/spmv_fp64_n2e24_traffic.json), not measured in this run",
+                     "launch_ms": {"mean": round(sp["ms"], 4), "median": round(sp["ms_median"], 4),
+                                   "min": round(sp["ms_min"], 4), "max": round(sp["ms_max"], 4), "n": args.steps},
+                     "achieved_at_median_launch": round(sp["bytes"] / (sp["ms_median"] * 1e-3) / 1e9, 2)},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
                          else None),
         "spmv_fp32": sp.get("fp32"),
@@ -1093,4 +1344,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "viability-sweep":
+        viability_sweep_main(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "rocprof-summary":
+        rocprof_summary_main(sys.argv[2:])
+    else:
+        main()

@@ -78,6 +78,8 @@ SIGNATURES = {
     "dq_hamsim_host": (c_int, [c_void_p, P(c_double), P(c_double), c_double, c_uint64, c_int, P(c_double),
                                P(c_double)]),
     "dq_hamsim_device": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_uint64, c_int, c_uintptr]),
+    "dq_tensordot": (c_int, [c_void_p, c_int, P(c_uint32), c_void_p, c_int, P(c_uint32), c_int, P(c_uint32),
+                             P(c_uint32), P(c_void_p), P(c_uint32), P(c_int)]),
     "dq_hamsim_device_f64state": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_uint64, c_int, c_uintptr]),
     "dq_spmm_dia_device": (c_int, [c_uintptr, c_uintptr, c_int, c_int64, c_int64, c_uintptr, c_int64, c_uintptr, c_int,
                                    c_uintptr]),

@@ -656,7 +656,6 @@ extern "C" int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, con
         if (A->n() && (!re_dev || !im_dev)) throw ArgError("dq_devmat_set_packed_device: NULL device array");
         hipStream_t s = as_stream(stream);
         const size_t eb = elem_bytes(A->precision);
-        A->zmap_valid = false;  // the values change: the zero-block map of the last tuning no longer holds
         int64_t src = 0;
         for (int k = 0; k < A->n(); ++k) {
             if (A->len[k]) {
@@ -669,6 +668,8 @@ extern "C" int dq_devmat_set_packed_device(dq_devmat *A, const void *re_dev, con
             }
             src += A->len[k];
         }
+        // the values changed: the zero-block map a tuning kept is recomputed for them, in its own buffer
+        refresh_zero_map(*A, s);
     });
 }
 
@@ -837,5 +838,8 @@ extern "C" int dq_trace(const dq_devmat *A, double *re, double *im) {
 }
 
 extern "C" int dq_release_cached_memory(void) {
-    return guard([&] { dq::release_cached_memory(); });
+    return guard([&] {
+        dq::release_dryrun_cache();
+        dq::release_cached_memory();
+    });
 }

@@ -82,6 +82,7 @@ void *dev_alloc(size_t bytes);          // device memory (a cached block or hipM
 void dev_free(void *p);                 // back to the cache (after the device is idle) or hipFree; never throws
 void set_alloc_cache_budget(int64_t mib);  // bytes the cache of freed blocks may hold; -1: an eighth of the device
 void release_cached_memory();           // hipFree every cached block
+void release_dryrun_cache();            // drop the product dry-runs kept for reuse (product.hip)
 
 struct DevBuf {
     void *p = nullptr;
@@ -150,7 +151,7 @@ struct dq_devmat {
     // library default): rows per lane, non-temporal loads, load schedule, LDS x window.
     int skip_rpt = 0, skip_nt = -1, skip_pipe = -1, skip_lds = -1;
     // The zero-block map (256-row blocks, zmap_words words per diagonal) dq_spmv_tune keeps when the matrix has
-    // all-zero blocks; zmap_valid is cleared by every call that rewrites the values (dq_devmat_set_packed_device).
+    // all-zero blocks; a call that rewrites the values (dq_devmat_set_packed_device) recomputes it in place.
     dq::DevBuf zmap;
     int64_t zmap_words = 0;
     bool zmap_valid = false;
@@ -230,6 +231,9 @@ void kron_apply(const dq_devmat &G, int before, int after, const void *xr, const
 
 // y = A x for device vectors of the matrix precision; col band [col_lo, col_hi) and accumulate for multi-pass use.
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s);
+
+// Recompute the zero-block map dq_spmv_tune keeps in A's handle (if any) after A's values were rewritten on s.
+void refresh_zero_map(dq_devmat &A, hipStream_t s);
 
 // Whether the LDS x window is the untuned default for A at rows-per-lane rpt (some off-diagonals within its halo).
 bool lds_window_pays(const dq_devmat &A, int rpt);

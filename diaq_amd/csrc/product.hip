@@ -20,6 +20,8 @@
 // in parallel and leave each pair's row range and base pointers in LDS, then the pair loop reads them from there
 // (no chain of dependent metadata loads per pair).
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <numeric>
 
 #include "dia_rows.hpp"
@@ -100,26 +102,46 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
 
 // The dry-run depends only on the shape and the two offset lists. A Taylor series multiplies operands with the same
 // offsets term after term once the terms reach every diagonal they can (TFIM-12: terms 8 to 14, 8191 x 25 offsets,
-// ~1.2 ms of host time each), so each thread keeps its last dry-run and reuses it when the inputs are equal.
-static const DryRun &dryrun_reused(uint32_t rows, uint32_t cols, const std::vector<int32_t> &oa,
-                                   const std::vector<int32_t> &ob) {
-    struct Last {
-        bool valid = false;
-        uint32_t rows = 0, cols = 0;
-        std::vector<int32_t> oa, ob;
-        DryRun d;
-    };
-    thread_local Last last;
-    if (!(last.valid && last.rows == rows && last.cols == cols && last.oa == oa && last.ob == ob)) {
-        last.valid = false;
-        last.d = dryrun(rows, cols, oa, ob);
-        last.rows = rows;
-        last.cols = cols;
-        last.oa = oa;
-        last.ob = ob;
-        last.valid = true;
-    }
-    return last.d;
+// ~1.2 ms of host time each), so each thread keeps its last dry-run and reuses it when the inputs are equal. Only
+// dry-runs of at most DRYRUN_CACHE_PAIRS slot pairs are kept (their lists take ~16 B per pair), and
+// dq_release_cached_memory drops the kept ones: the calling thread's at once, other threads' at their next product.
+constexpr int64_t DRYRUN_CACHE_PAIRS = int64_t(1) << 22;
+static std::atomic<uint64_t> g_dryrun_generation{0};
+
+struct DryRunCache {
+    uint64_t generation = 0;
+    uint32_t rows = 0, cols = 0;
+    std::vector<int32_t> oa, ob;
+    std::shared_ptr<const DryRun> d;
+};
+
+static DryRunCache &dryrun_cache() {
+    thread_local DryRunCache c;
+    return c;
+}
+
+void release_dryrun_cache() {
+    g_dryrun_generation.fetch_add(1);
+    DryRunCache &c = dryrun_cache();
+    c.d.reset();
+    c.oa = std::vector<int32_t>();
+    c.ob = std::vector<int32_t>();
+}
+
+static std::shared_ptr<const DryRun> dryrun_reused(uint32_t rows, uint32_t cols, const std::vector<int32_t> &oa,
+                                                   const std::vector<int32_t> &ob) {
+    DryRunCache &c = dryrun_cache();
+    const uint64_t gen = g_dryrun_generation.load();
+    if (c.d && c.generation == gen && c.rows == rows && c.cols == cols && c.oa == oa && c.ob == ob) return c.d;
+    auto d = std::make_shared<const DryRun>(dryrun(rows, cols, oa, ob));
+    const bool keep = static_cast<int64_t>(oa.size()) * static_cast<int64_t>(ob.size()) <= DRYRUN_CACHE_PAIRS;
+    c.d = keep ? d : nullptr;
+    c.generation = gen;
+    c.rows = rows;
+    c.cols = cols;
+    c.oa = keep ? oa : std::vector<int32_t>();
+    c.ob = keep ? ob : std::vector<int32_t>();
+    return d;
 }
 
 constexpr int PROD_BATCH = 256;
@@ -275,7 +297,8 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     if (A.cols != B.rows) throw ArgError("diaQMatrix product_cuda: dimension mismatch.");
     if (A.precision != B.precision) throw ArgError("product: operands have different precisions");
     if (allow_dense && dense_product_pays(A, B)) return product_dense(A, B, divisor, s);
-    const DryRun &d = dryrun_reused(A.rows, B.cols, A.offs, B.offs);
+    const std::shared_ptr<const DryRun> dp = dryrun_reused(A.rows, B.cols, A.offs, B.offs);
+    const DryRun &d = *dp;
     // every stored element of C is written by the kernel; only the gaps between diagonals are zeroed
     auto C = make_devmat(A.rows, B.cols, d.res.data(), static_cast<int>(d.res.size()), A.precision, s, false);
     const int nc = C->n();

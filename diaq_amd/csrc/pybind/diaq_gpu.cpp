@@ -350,6 +350,31 @@ static std::shared_ptr<HostDia> product_host(const HostDia &A, const HostDia &B)
     return download(DeviceMatrix(q));
 }
 
+// tensordot on device matrices (dq_tensordot); the reference's argument errors are std::invalid_argument, which
+// pybind11 raises as ValueError (src/tensordotKernels.cpp:24-61)
+static std::shared_ptr<DeviceMatrix> tensordot_dev(const DeviceMatrix &A, const std::vector<unsigned> &ta,
+                                                   const DeviceMatrix &B, const std::vector<unsigned> &tb,
+                                                   const std::vector<std::vector<unsigned>> &axes) {
+    if (axes.size() != 2 || axes[0].size() != axes[1].size())
+        throw std::invalid_argument("tensordot: axes must be two lists of the same length");
+    std::vector<uint32_t> sa(ta.begin(), ta.end()), sb(tb.begin(), tb.end());
+    std::vector<uint32_t> xa(axes[0].begin(), axes[0].end()), xb(axes[1].begin(), axes[1].end());
+    std::vector<uint32_t> out_shape(std::max<size_t>(2, sa.size() + sb.size() + 4));
+    int out_rank = 0;
+    dq_devmat *p = nullptr;
+    const int rc = dq_tensordot(A.h.get(), (int)sa.size(), sa.data(), B.h.get(), (int)sb.size(), sb.data(),
+                                (int)xa.size(), xa.data(), xb.data(), &p, out_shape.data(), &out_rank);
+    if (rc == DQ_ERR_ARG) throw std::invalid_argument(dq_last_error());
+    check(rc);
+    return wrap(p, std::vector<unsigned>(out_shape.begin(), out_shape.begin() + out_rank));
+}
+
+static std::shared_ptr<HostDia> tensordot_host(const HostDia &A, const HostDia &B,
+                                               const std::vector<std::vector<unsigned>> &axes) {
+    auto dA = upload(A), dB = upload(B);
+    return download(*tensordot_dev(*dA, A.shape, *dB, B.shape, axes));
+}
+
 static std::shared_ptr<HostDia> pow_host(const HostDia &A, long long n) {
     if (n < 0) throw std::runtime_error("pow_cuda: negative exponent not supported");
     auto D = upload(A);
@@ -703,6 +728,27 @@ PYBIND11_MODULE(diaq_gpu, m) {
     // kernels on host matrices (each call uploads, as the reference's host API does)
     m.def("product", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostDia> B) { return product_host(*A, *B); });
     m.def("trace", [](std::shared_ptr<HostDia> A) { return trace_host(*A); });
+    // tensordot and the reference's numpy overloads for quimb (include/pythonWrappers.cpp:270-330); the contraction
+    // runs on the GPU
+    using Axes = std::vector<std::vector<unsigned>>;
+    m.def("tensordot", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostDia> B, Axes axes) {
+        return tensordot_host(*A, *B, axes);
+    }, py::arg(), py::arg(), py::arg("axes") = Axes{});
+    m.def("tensordot", [](py::array_t<cplx, py::array::c_style | py::array::forcecast> a,
+                          py::array_t<cplx, py::array::c_style | py::array::forcecast> b, Axes axes) {
+        std::vector<unsigned> sa(a.shape(), a.shape() + a.ndim()), sb(b.shape(), b.shape() + b.ndim());
+        return tensordot_host(*from_numpy_shape(a.data(), sa), *from_numpy_shape(b.data(), sb), axes);
+    }, py::arg("arr1"), py::arg("arr2"), py::arg("axes") = Axes{});
+    m.def("tensordot", [](std::shared_ptr<HostDia> A, py::array_t<cplx, py::array::c_style | py::array::forcecast> b,
+                          Axes axes) {
+        std::vector<unsigned> sb(b.shape(), b.shape() + b.ndim());
+        return tensordot_host(*A, *from_numpy_shape(b.data(), sb), axes);
+    }, py::arg("mat"), py::arg("arr"), py::arg("axes") = Axes{});
+    m.def("tensordot", [](py::array_t<cplx, py::array::c_style | py::array::forcecast> a, std::shared_ptr<HostDia> B,
+                          Axes axes) {
+        std::vector<unsigned> sa(a.shape(), a.shape() + a.ndim());
+        return tensordot_host(*from_numpy_shape(a.data(), sa), *B, axes);
+    }, py::arg("arr"), py::arg("mat"), py::arg("axes") = Axes{});
     m.def("spMV", [](std::shared_ptr<HostDia> A, std::shared_ptr<HostVector> x) {
         auto y = std::make_shared<HostVector>(A->matrix_shape[0]);
         spmv_into(*A, *x, *y);
@@ -779,8 +825,8 @@ PYBIND11_MODULE(diaq_gpu, m) {
 
     // device-resident matrices
     py::class_<DeviceMatrix, std::shared_ptr<DeviceMatrix>>(m, "DeviceMatrix",
-        "A diaQMatrix in GPU memory (uploaded once). spMV(DeviceMatrix, vector) copies only the vectors; "
-        "product(DeviceMatrix, DeviceMatrix) returns a DeviceMatrix without leaving the GPU; to_host() downloads.")
+        "A DIA matrix resident in HBM. The matrix is uploaded once; spMV on it moves only the vectors, products "
+        "of two of them stay on the device, and to_host() brings a copy back as a diaQMatrix.")
         .def(py::init([](std::shared_ptr<HostDia> A, const std::string &precision) {
             return upload(*A, parse_precision(precision));
         }), py::arg("A"), py::arg("precision") = "fp64")
@@ -820,7 +866,7 @@ PYBIND11_MODULE(diaq_gpu, m) {
                                "the dq_devmat pointer for the C ABI");
     m.def("to_device", [](std::shared_ptr<HostDia> A, const std::string &precision) {
         return upload(*A, parse_precision(precision));
-    }, py::arg("A"), py::arg("precision") = "fp64", "Upload a diaQMatrix to the GPU (a DeviceMatrix).");
+    }, py::arg("A"), py::arg("precision") = "fp64", "Copy a host diaQMatrix into HBM and return the DeviceMatrix holding it.");
     m.def("spMV", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<HostVector> x) { return spmv_dev(*A, *x); },
           py::arg(), py::arg());
     m.def("pauli_sum_device", [](int n_qubits, std::vector<uint64_t> x_masks, std::vector<uint64_t> z_masks,
@@ -858,6 +904,10 @@ PYBIND11_MODULE(diaq_gpu, m) {
         return product_dev(*A, *B);
     }, py::arg(), py::arg());
     m.def("trace", [](std::shared_ptr<DeviceMatrix> A) { return trace_dev(*A); });
+    m.def("tensordot", [](std::shared_ptr<DeviceMatrix> A, std::shared_ptr<DeviceMatrix> B, Axes axes) {
+        return tensordot_dev(*A, A->tensor_shape, *B, B->tensor_shape, axes);
+    }, py::arg(), py::arg(), py::arg("axes") = Axes{},
+       "np.tensordot of two device-resident tensors; the result stays on the device with its tensor shape");
     // pow / expm / expm_negative on a DeviceMatrix: the result stays on the device (the host-matrix forms above return
     // a host diaQMatrix, as the reference's do, which for a near-dense result means downloading it)
     m.def("pow", [](std::shared_ptr<DeviceMatrix> A, long long n) {
@@ -902,15 +952,17 @@ PYBIND11_MODULE(diaq_gpu, m) {
                             uintptr_t stream) {
         check(dq_spmv_device(A->h.get(), (const void *)xr, (const void *)xi, (void *)yr, (void *)yi, stream));
     }, py::arg("A"), py::arg("x_real"), py::arg("x_imag"), py::arg("y_real"), py::arg("y_imag"), py::arg("stream") = 0,
-       "y = A x with x, y as GPU arrays of float64 (float32 for an fp32 DeviceMatrix), real and imaginary parts "
-       "(e.g. torch data pointers); y is overwritten; asynchronous on `stream`.");
+       "Device-pointer SpMV: x and y are the split real and imaginary device arrays (float64; float32 for an fp32 "
+       "DeviceMatrix) given as integer addresses, e.g. torch data_ptr(). y is written, not accumulated, and the "
+       "call returns before the kernel ends (it runs on `stream`).");
     m.def("spmm_dia_device", [](uintptr_t offsets, uintptr_t vals, int n_diags, int64_t n_rows, int64_t n_cols,
                                 uintptr_t B, int64_t n_right, uintptr_t C, bool is_complex, uintptr_t stream) {
         check(dq_spmm_dia_device(offsets, vals, n_diags, n_rows, n_cols, B, n_right, C, is_complex ? 1 : 0, stream));
     }, py::arg("offsets"), py::arg("vals"), py::arg("n_diags"), py::arg("n_rows"), py::arg("n_cols"), py::arg("B"),
        py::arg("n_right"), py::arg("C"), py::arg("is_complex"), py::arg("stream") = 0,
-       "C = A * B, A in row-aligned DIA format (offsets int64[n_diags], vals[n_diags * n_rows]); B (n_cols x n_right) "
-       "and C (n_rows x n_right) row-major; double or complex double; all pointers on the GPU.");
+       "Dense product C = A B on device pointers. A is stored row-aligned: vals[k * n_rows + r] holds A[r, r + "
+       "offsets[k]] (offsets int64). B has n_cols rows and C n_rows rows, both with n_right columns in row-major "
+       "order, real or complex double. C is overwritten.");
     m.def("hamiltonian_simulation_cuda", [](std::shared_ptr<HostDia> H, std::shared_ptr<HostVector> psi0, double dt,
                                             unsigned long r, int max_iterations, double scale_cutoff,
                                             double tolerance) {

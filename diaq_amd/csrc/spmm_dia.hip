@@ -436,7 +436,7 @@ struct ViabilityRule {
     int64_t min_budget, rows_per_diagonal, wide_right, wide_budget;
 };
 
-// The reference's constants (an A100 sweep) and the ones fitted on MI355X by diaq_amd.viability_sweep over the same
+// The reference's constants (an A100 sweep) and the ones fitted on MI355X by `bench.py viability-sweep` over the same
 // 360-case grid: no case passing them ran slower in DIA than hipSPARSE's CSR x dense, and they keep 228 of the 305
 // cases where DIA was faster (the reference's constants keep 63).
 constexpr ViabilityRule REFERENCE_RULE{0.9, 16, 32, 32, 8};

@@ -189,8 +189,17 @@ bool lds_window_for(const dq_devmat &A, int rpt) {
     return lds_window_pays(A, rpt);
 }
 
+// A complete SpMV schedule, for callers that choose one explicitly (the tuner's candidates).
+struct SpmvSched {
+    int rpt;
+    bool nt, pipe, lds;
+    int xcd;
+};
+
+// y = A x in n_bands column passes (0: automatic), skipping A's all-zero blocks when zm is given. The schedule is
+// `force` if given, else the process-wide knobs, else A's tuned choice, else the defaults.
 void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, int n_bands,
-                 hipStream_t s, const uint32_t *zm = nullptr, int64_t zwords = 0) {
+                 hipStream_t s, const uint32_t *zm = nullptr, int64_t zwords = 0, const SpmvSched *force = nullptr) {
     if (A.rows == 0) return;
     if (A.n() == 0) {  // no diagonals: y = 0 (a fill kernel, safe inside graph capture)
         const unsigned g = blocks_for(A.rows, 256, 4096);
@@ -205,10 +214,11 @@ void spmv_banded(const dq_devmat &A, const void *xr, const void *xi, void *yr, v
     }
     if (n_bands <= 0) n_bands = auto_bands(A);
     if (n_bands > static_cast<int>(A.cols)) n_bands = std::max<int>(1, A.cols);
-    const int rpt = zm ? skip_rpt_for(A) : auto_rpt(A);
-    const bool nt = zm ? skip_nt_for(A) : auto_nt(A), pipe = zm ? skip_pipe_for(A) : auto_pipe(A),
-               lds = zm ? skip_lds_for(A, rpt) : lds_window_for(A, rpt);
-    const int xcd = xcd_tiles_for(A);
+    const int rpt = force ? force->rpt : zm ? skip_rpt_for(A) : auto_rpt(A);
+    const bool nt = force ? force->nt : zm ? skip_nt_for(A) : auto_nt(A);
+    const bool pipe = force ? force->pipe : zm ? skip_pipe_for(A) : auto_pipe(A);
+    const bool lds = force ? force->lds : zm ? skip_lds_for(A, rpt) : lds_window_for(A, rpt);
+    const int xcd = force ? force->xcd : xcd_tiles_for(A);
     const int64_t cols = A.cols;
     for (int b = 0; b < n_bands; ++b) {
         const int64_t c0 = cols * b / n_bands, c1 = cols * (b + 1) / n_bands;
@@ -275,6 +285,14 @@ static void ntimes_zero_map(const dq_devmat &A, int64_t zwords, uint32_t *z, hip
     DQ_LAUNCH_CHECK();
 }
 
+// After A's values were rewritten on stream s: recompute the zero-block map the handle keeps into the same buffer,
+// ordered after the rewrite. A graph captured earlier that reads the handle's map then sees the new map on its next
+// replay, and later SpMVs skip exactly the blocks that are zero now (none, if every block became nonzero).
+void refresh_zero_map(dq_devmat &A, hipStream_t s) {
+    if (!A.zmap_valid || !A.zmap.p) return;
+    ntimes_zero_map(A, A.zmap_words, A.zmap.as<uint32_t>(), s);
+}
+
 // y = A x: the schedule of A, skipping its all-zero blocks when the handle holds their map (see handle_map).
 void spmv(const dq_devmat &A, const void *xr, const void *xi, void *yr, void *yi, hipStream_t s) {
     const uint32_t *zm = handle_map(A);
@@ -324,7 +342,9 @@ extern "C" int dq_set_tuning(const char *key, int value) {
 
 // Time every (load schedule, rows per lane, non-temporal, LDS window) candidate on scratch vectors and keep the fastest
 // in the matrix handle; later SpMVs with this matrix use it (unless a process-wide knob overrides it). The LDS window
-// is only tried when some off-diagonal lies within its halo.
+// is only tried when some off-diagonal lies within its halo. The candidates are passed to the kernels explicitly, so
+// SpMVs of other matrices meanwhile keep their own schedules; calls with this matrix must not overlap its tuning
+// (the handle's schedule and zero-block map change during it).
 extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms) {
     return guard([&] {
         if (!A) throw ArgError("matrix is NULL");
@@ -338,77 +358,67 @@ extern "C" int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *
         hipEvent_t e0, e1;
         DQ_HIP(hipEventCreate(&e0));
         DQ_HIP(hipEventCreate(&e1));
-        float best = 1e30f;
-        int brpt = 8, bnt = 1, bpipe = 1, blds = 0, bxcd = 0;
-        const int saved_rpt = g_spmv_rpt, saved_nt = g_spmv_nt, saved_pipe = g_spmv_pipe, saved_lds = g_spmv_lds,
-                  saved_xcd = g_spmv_xcd;
-        g_spmv_rpt = 0; g_spmv_nt = -1; g_spmv_pipe = -1; g_spmv_lds = -1; g_spmv_xcd = -1;
+        struct Events {
+            hipEvent_t a, b;
+            ~Events() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+        } ev{e0, e1};
         const uint32_t *tz = nullptr;  // the zero-block map while the skip schedule is timed
         int64_t tzw = 0;
-        auto time_it = [&]() {
-            spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw);  // warm
+        auto time_it = [&](const SpmvSched &c) {
+            spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw, &c);  // warm
             DQ_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw);
+            for (int r = 0; r < repeats; ++r) spmv_banded(*A, xr.p, xi.p, yr.p, yi.p, 1, s, tz, tzw, &c);
             DQ_HIP(hipEventRecord(e1, s));
             DQ_HIP(hipEventSynchronize(e1));
             float ms = 0.f;
             DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
             return ms / repeats;
         };
-        try {
-            A->tuned_xcd = 0;
-            for (int lds = 0; lds <= 1; ++lds)
+        float best = 1e30f;
+        SpmvSched win{8, true, true, false, 0};
+        for (int lds = 0; lds <= 1; ++lds)
             for (int pipe = 0; pipe <= 1; ++pipe)
                 for (int rpt : {2, 4, 8, 16})
                     for (int nt = 0; nt <= 1; ++nt) {
                         if (lds && !lds_window_pays(*A, rpt) && !lds_window_pays(*A, 16)) continue;
-                        A->tuned_rpt = rpt; A->tuned_nt = nt; A->tuned_pipe = pipe; A->tuned_lds = lds;
-                        const float ms = time_it();
-                        if (ms < best) { best = ms; brpt = rpt; bnt = nt; bpipe = pipe; blds = lds; }
+                        const SpmvSched c{rpt, nt == 1, pipe == 1, lds == 1, 0};
+                        const float ms = time_it(c);
+                        if (ms < best) { best = ms; win = c; }
                     }
-            // second stage: the winner with the XCD-contiguous tile order
-            A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds; A->tuned_xcd = 1;
-            const float ms = time_it();
-            if (ms < best) { best = ms; bxcd = 1; }
-            A->tuned_xcd = bxcd;
-            // third stage: when A has all-zero blocks, keep their map in the handle and time the schedules that
-            // skip them (used by A's FP64 SpMVs and by spMV_ntimes)
-            A->skip_rpt = brpt; A->skip_nt = bnt; A->skip_pipe = bpipe; A->skip_lds = blds;
-            A->zmap_valid = false;
-            if (g_spmv_skip) {
-                const size_t zb = ntimes_map_bytes(*A, tzw);
-                if (A->zmap.bytes != zb) A->zmap = DevBuf(zb);
-                A->zmap_words = tzw;
-                ntimes_zero_map(*A, tzw, A->zmap.as<uint32_t>(), s);
-                if (has_zero_blocks(*A, A->zmap.as<uint32_t>(), tzw, s)) {
-                    A->zmap_valid = true;
-                    tz = A->zmap.as<uint32_t>();
-                    float zbest = 1e30f;
-                    int zr = 4, znt = 1, zp = 0, zl = 0;
-                    for (int lds = 0; lds <= 1; ++lds)
-                        for (int pipe = 0; pipe <= 1; ++pipe)
-                            for (int rpt : {1, 2, 4, 8, 16})
-                                for (int nt = 0; nt <= 1; ++nt) {
-                                    if (lds && !lds_window_pays(*A, rpt)) continue;
-                                    A->skip_rpt = rpt; A->skip_nt = nt; A->skip_pipe = pipe; A->skip_lds = lds;
-                                    const float zms = time_it();
-                                    if (zms < zbest) { zbest = zms; zr = rpt; znt = nt; zp = pipe; zl = lds; }
-                                }
-                    tz = nullptr;
-                    A->skip_rpt = zr; A->skip_nt = znt; A->skip_pipe = zp; A->skip_lds = zl;
-                }
+        // second stage: the winner with the XCD-contiguous tile order
+        SpmvSched wx = win;
+        wx.xcd = 1;
+        const float msx = time_it(wx);
+        if (msx < best) { best = msx; win = wx; }
+        A->tuned_rpt = win.rpt; A->tuned_nt = win.nt; A->tuned_pipe = win.pipe; A->tuned_lds = win.lds;
+        A->tuned_xcd = win.xcd;
+        // third stage: when A has all-zero blocks, keep their map in the handle and time the schedules that skip
+        // them (used by A's FP64 SpMVs and by spMV_ntimes)
+        A->skip_rpt = win.rpt; A->skip_nt = win.nt; A->skip_pipe = win.pipe; A->skip_lds = win.lds;
+        A->zmap_valid = false;
+        if (g_spmv_skip) {
+            const size_t zb = ntimes_map_bytes(*A, tzw);
+            if (A->zmap.bytes != zb) A->zmap = DevBuf(zb);
+            A->zmap_words = tzw;
+            ntimes_zero_map(*A, tzw, A->zmap.as<uint32_t>(), s);
+            if (has_zero_blocks(*A, A->zmap.as<uint32_t>(), tzw, s)) {
+                tz = A->zmap.as<uint32_t>();
+                float zbest = 1e30f;
+                SpmvSched zw{4, true, false, false, win.xcd};
+                for (int lds = 0; lds <= 1; ++lds)
+                    for (int pipe = 0; pipe <= 1; ++pipe)
+                        for (int rpt : {1, 2, 4, 8, 16})
+                            for (int nt = 0; nt <= 1; ++nt) {
+                                if (lds && !lds_window_pays(*A, rpt)) continue;
+                                const SpmvSched c{rpt, nt == 1, pipe == 1, lds == 1, win.xcd};
+                                const float zms = time_it(c);
+                                if (zms < zbest) { zbest = zms; zw = c; }
+                            }
+                tz = nullptr;
+                A->skip_rpt = zw.rpt; A->skip_nt = zw.nt; A->skip_pipe = zw.pipe; A->skip_lds = zw.lds;
+                A->zmap_valid = true;
             }
-        } catch (...) {
-            g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
-            g_spmv_xcd = saved_xcd;
-            (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-            throw;
         }
-        g_spmv_rpt = saved_rpt; g_spmv_nt = saved_nt; g_spmv_pipe = saved_pipe; g_spmv_lds = saved_lds;
-        g_spmv_xcd = saved_xcd;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        A->tuned_rpt = brpt; A->tuned_nt = bnt; A->tuned_pipe = bpipe; A->tuned_lds = blds; A->tuned_xcd = bxcd;
         if (best_ms) *best_ms = best;
     });
 }
@@ -492,64 +502,87 @@ extern "C" int dq_spmv_host(const dq_devmat *A, const double *xr, const double *
     });
 }
 
-extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int n,
-                                     void *work_r, void *work_i, uintptr_t stream) {
-    return guard([&] {
-        if (!A) throw ArgError("matrix is NULL");
-        if (n < 0) throw ArgError("spmv_ntimes_cuda: num_times must be >= 0");
-        if (!xr || !xi || !yr || !yi) throw ArgError("spmv_ntimes_device: NULL vector");
-        const int64_t M = A->rows, N = A->cols;
-        const size_t eb = elem_bytes(A->precision);
-        hipStream_t s = as_stream(stream);
-        if (n == 0) {
-            if (M != N)
-                throw ArgError("spmv_ntimes_cuda: num_times==0 requires square matrix (M==N) to copy x -> y");
-            if (xr != yr) DQ_HIP(hipMemcpyAsync(yr, xr, M * eb, hipMemcpyDeviceToDevice, s));
-            if (xi != yi) DQ_HIP(hipMemcpyAsync(yi, xi, M * eb, hipMemcpyDeviceToDevice, s));
-            return;
-        }
-        if (n > 1 && M != N)
-            throw ArgError("spmv_ntimes_cuda: repeated SpMV requires square matrix (M==N) for num_times > 1");
-        const bool in_place = xr == yr || xi == yi;
-        if (in_place && (xr != yr || xi != yi)) throw ArgError("spmv_ntimes_device: x and y overlap in one part only");
-        if ((n > 1 || in_place) && (!work_r || !work_i))
-            throw ArgError("spmv_ntimes_device: work vectors are needed for num_times > 1 or x == y");
-        // destinations alternate between y and the work vectors so that the last product lands in y; with x == y and
-        // an odd count the first product must not overwrite x, so the order flips and a copy ends the sequence
-        const bool flip = in_place && (n % 2 == 1);
-        // the zero blocks of A, skipped by every product of the sequence (A cannot change during the call): the map
-        // dq_spmv_tune kept in the handle, or else, for two or more products, one made here in stream-ordered memory,
-        // so the call stays asynchronous. While the stream is being captured into a graph no map is made (a replay
-        // must not read memory this call has given back).
-        int64_t zwords = 0;
-        uint32_t *zmap = nullptr;
-        const uint32_t *zuse = sequence_map(*A, n);
-        if (zuse) zwords = A->zmap_words;
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        DQ_HIP(hipStreamIsCapturing(s, &cap));
-        if (!zuse && ntimes_uses_map(*A, n) && cap == hipStreamCaptureStatusNone) {
-            if (hipMallocAsync(reinterpret_cast<void **>(&zmap), ntimes_map_bytes(*A, zwords), s) != hipSuccess) {
-                (void)hipGetLastError();  // no stream-ordered memory: run without the map
-                zmap = nullptr;
-            } else {
-                ntimes_zero_map(*A, zwords, zmap, s);
-                zuse = zmap;
+// y = A^n x on device vectors (src/spMV_ntimes_gpu.cu:61-263 on resident data). `fixed_map` (may be null) is
+// device memory of fixed_words words the call may use for A's zero-block map instead of allocating one: the
+// allocation-free entry point passes its static scratch array; then the map is made even while the stream is being
+// captured (the array outlives every replay), and a map that would not fit is an error.
+static void ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int n, void *work_r,
+                          void *work_i, hipStream_t s, uint32_t *fixed_map, int64_t fixed_words) {
+    if (!A) throw ArgError("matrix is NULL");
+    if (n < 0) throw ArgError("spmv_ntimes_cuda: num_times must be >= 0");
+    if (!xr || !xi || !yr || !yi) throw ArgError("spmv_ntimes_device: NULL vector");
+    const int64_t M = A->rows, N = A->cols;
+    const size_t eb = elem_bytes(A->precision);
+    if (n == 0) {
+        if (M != N) throw ArgError("spmv_ntimes_cuda: num_times==0 requires square matrix (M==N) to copy x -> y");
+        if (xr != yr) DQ_HIP(hipMemcpyAsync(yr, xr, M * eb, hipMemcpyDeviceToDevice, s));
+        if (xi != yi) DQ_HIP(hipMemcpyAsync(yi, xi, M * eb, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    if (A->n() == 0) {  // no diagonals: y = 0 for any num_times >= 1 (src/spMV_ntimes_gpu.cu:98-105)
+        spmv_banded(*A, xr, xi, yr, yi, 0, s);
+        return;
+    }
+    if (n > 1 && M != N)
+        throw ArgError("spmv_ntimes_cuda: repeated SpMV requires square matrix (M==N) for num_times > 1");
+    const bool in_place = xr == yr || xi == yi;
+    if (in_place && (xr != yr || xi != yi)) throw ArgError("spmv_ntimes_device: x and y overlap in one part only");
+    if ((n > 1 || in_place) && (!work_r || !work_i))
+        throw ArgError("spmv_ntimes_device: work vectors are needed for num_times > 1 or x == y");
+    // destinations alternate between y and the work vectors so that the last product lands in y; with x == y and an
+    // odd count the first product must not overwrite x, so the order flips and a copy ends the sequence
+    const bool flip = in_place && (n % 2 == 1);
+    // the zero blocks of A, skipped by every product of the sequence (A cannot change during the call): the map
+    // dq_spmv_tune kept in the handle, or else, for two or more products, one made here -- in the caller's fixed
+    // memory, or in stream-ordered memory so the call stays asynchronous. Without fixed memory no map is made while
+    // the stream is being captured into a graph (a replay must not read memory this call has given back).
+    int64_t zwords = 0;
+    uint32_t *zmap = nullptr;
+    const uint32_t *zuse = sequence_map(*A, n);
+    if (zuse) zwords = A->zmap_words;
+    if (!zuse && ntimes_uses_map(*A, n)) {
+        const size_t bytes = ntimes_map_bytes(*A, zwords);
+        if (fixed_map) {
+            if (static_cast<int64_t>(bytes / sizeof(uint32_t)) > fixed_words)
+                throw ArgError("spmv_ntimes_device_noalloc: the zero-block map of this matrix needs " +
+                               std::to_string(bytes) + " bytes, more than the " +
+                               std::to_string(fixed_words * sizeof(uint32_t)) +
+                               " bytes of the library's static scratch (tune the matrix with dq_spmv_tune to keep "
+                               "its map in the handle, or use dq_spmv_ntimes_device)");
+            ntimes_zero_map(*A, zwords, fixed_map, s);
+            zuse = fixed_map;
+        } else {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            DQ_HIP(hipStreamIsCapturing(s, &cap));
+            if (cap == hipStreamCaptureStatusNone) {
+                if (hipMallocAsync(reinterpret_cast<void **>(&zmap), bytes, s) != hipSuccess) {
+                    (void)hipGetLastError();  // no stream-ordered memory: run without the map
+                    zmap = nullptr;
+                } else {
+                    ntimes_zero_map(*A, zwords, zmap, s);
+                    zuse = zmap;
+                }
             }
         }
-        const void *sr = xr, *si = xi;
-        for (int t = 0; t < n; ++t) {
-            const bool to_y = (((n - 1 - t) % 2) == 0) != flip;
-            void *dr = to_y ? yr : work_r, *di = to_y ? yi : work_i;
-            spmv_banded(*A, sr, si, dr, di, 0, s, zuse, zuse ? zwords : 0);
-            sr = dr;
-            si = di;
-        }
-        if (zmap) DQ_HIP(hipFreeAsync(zmap, s));
-        if (flip) {
-            DQ_HIP(hipMemcpyAsync(yr, work_r, M * eb, hipMemcpyDeviceToDevice, s));
-            DQ_HIP(hipMemcpyAsync(yi, work_i, M * eb, hipMemcpyDeviceToDevice, s));
-        }
-    });
+    }
+    const void *sr = xr, *si = xi;
+    for (int t = 0; t < n; ++t) {
+        const bool to_y = (((n - 1 - t) % 2) == 0) != flip;
+        void *dr = to_y ? yr : work_r, *di = to_y ? yi : work_i;
+        spmv_banded(*A, sr, si, dr, di, 0, s, zuse, zuse ? zwords : 0);
+        sr = dr;
+        si = di;
+    }
+    if (zmap) DQ_HIP(hipFreeAsync(zmap, s));
+    if (flip) {
+        DQ_HIP(hipMemcpyAsync(yr, work_r, M * eb, hipMemcpyDeviceToDevice, s));
+        DQ_HIP(hipMemcpyAsync(yi, work_i, M * eb, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int n,
+                                     void *work_r, void *work_i, uintptr_t stream) {
+    return guard([&] { ntimes_device(A, xr, xi, yr, yi, n, work_r, work_i, as_stream(stream), nullptr, 0); });
 }
 
 extern "C" int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi,

@@ -146,8 +146,11 @@ int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi
 /* SpMV autotuner: times every (load schedule, rows per lane, non-temporal loads, LDS x window) candidate `repeats`
  * times on scratch vectors and keeps the fastest in the matrix handle, used by every later SpMV with A. When A has
  * all-zero 256-row blocks it also keeps their map in the handle and the fastest schedule that skips them: later FP64
- * SpMVs with A and spMV_ntimes skip those blocks (the same bits; dq_devmat_set_packed_device drops the map). best_ms
- * (may be NULL) receives the time of the schedule without the skip. No reference counterpart. */
+ * SpMVs with A and spMV_ntimes skip those blocks (the same bits; dq_devmat_set_packed_device recomputes the map for
+ * the new values, in place and in stream order, so graphs captured earlier stay correct). best_ms (may be NULL)
+ * receives the time of the schedule without the skip. The candidates are passed to the kernels directly: SpMVs of
+ * other matrices meanwhile keep their schedules, but calls with A itself must not overlap its tuning. No reference
+ * counterpart. */
 int dq_spmv_tune(dq_devmat *A, int repeats, uintptr_t stream, float *best_ms);
 
 /* The SpMV schedule A currently uses (after tuning or the defaults); any output pointer may be NULL. No reference
@@ -157,7 +160,8 @@ int dq_spmv_tuned_config(const dq_devmat *A, int *rows_per_lane, int *nontempora
 
 /* Return every cached device block to the runtime. The library keeps freed device blocks (up to a budget, by
  * default an eighth of the device memory; dq_set_tuning "alloc_cache_mb") to serve later allocations without
- * hipMalloc. No reference counterpart. */
+ * hipMalloc. Also drops the product dry-runs kept for reuse (the calling thread's at once, other threads' at their
+ * next product). No reference counterpart. */
 int dq_release_cached_memory(void);
 
 /* Process-wide tuning knobs for the autotuner and benchmarks: "spmv_rows_per_lane" (0 = automatic, or 1, 2, 4, 8,
@@ -257,6 +261,18 @@ int dq_frobenius_diff(const dq_devmat *A, const dq_devmat *B, double *out);
 
 /* Sum of the main diagonal. Replaces trace (src/matrixKernels.cpp:9-26). */
 int dq_trace(const dq_devmat *A, double *re, double *im);
+
+/* np.tensordot(A, B, axes) for DIA-stored tensors, on the device. A holds a tensor of rank rank_a and shape shape_a
+ * (rank_a = 0: the matrix itself, rank 2) in row-major order over its matrix positions; likewise B. The n_axes axes
+ * axes_a[k] of A are contracted with axes_b[k] of B. The result's axes are A's free axes, then B's, each in their
+ * order (out_shape, with room for rank_a + rank_b entries, and out_rank; a full contraction gives shape {1, 1}), as a
+ * free_A x free_B matrix whose exactly-zero diagonals are pruned. A and B are permuted into free x contracted and
+ * contracted x free matrices of their nonzeros by index-map kernels, then multiplied by dq_product_device. Replaces
+ * tensordot (src/tensordotKernels.cpp:21-74) with its transpose / matrix_reshape / product chain; errors as the
+ * reference's (axes of different lengths, an axis out of range, contracted sizes that differ). */
+int dq_tensordot(const dq_devmat *A, int rank_a, const uint32_t *shape_a, const dq_devmat *B, int rank_b,
+                 const uint32_t *shape_b, int n_axes, const uint32_t *axes_a, const uint32_t *axes_b, dq_devmat **out,
+                 uint32_t *out_shape, int *out_rank);
 
 /* ------------------------------------------------------------------------------------------------------------------
  * pow and SpME

@@ -510,16 +510,21 @@ def matrix_kron_identity_times_vector(G: Dia, n: int, x, y):
     return kron_apply(0, G, n, x, y)
 
 
-def dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize=16, index_bytes=8):
-    """Storage and diagonal budgets for DIA x dense (src/dia_viability.hpp:42-63)."""
+VIABILITY_CONSTANTS = dict(max_bytes_over_csr=0.9, min_budget=16, rows_per_diagonal=32, wide_right=32, wide_budget=8)
+
+
+def dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize=16, index_bytes=8, constants=None):
+    """Storage and diagonal budgets for DIA x dense (src/dia_viability.hpp:42-63). `constants` replaces the reference's
+    five (src/dia_viability.hpp:36-40, VIABILITY_CONSTANTS) in the same rule."""
+    c = VIABILITY_CONSTANTS if constants is None else constants
     stored = float(n_diags) * float(n_rows)
     dia_bytes = stored * itemsize + float(n_diags) * 8.0
     csr_bytes = float(nnz) * (itemsize + index_bytes) + float(n_rows + 1) * index_bytes
     density = float(nnz) / stored if stored > 0 else 0.0
-    storage_ok = n_diags > 0 and dia_bytes <= 0.9 * csr_bytes
-    budget = max(16, int(n_rows) // 32)
-    if n_right >= 32:
-        budget = min(budget, 8)
+    storage_ok = n_diags > 0 and dia_bytes <= c["max_bytes_over_csr"] * csr_bytes
+    budget = max(c["min_budget"], int(n_rows) // c["rows_per_diagonal"])
+    if n_right >= c["wide_right"]:
+        budget = min(budget, c["wide_budget"])
     diagonals_ok = n_diags <= budget
     viable = storage_ok and diagonals_ok
     if not storage_ok:
@@ -530,6 +535,28 @@ def dia_viability(n_rows, n_cols, n_diags, nnz, n_right, itemsize=16, index_byte
         reason = "viable"
     return dict(viable=viable, storage_ok=storage_ok, diagonals_ok=diagonals_ok, dia_bytes=dia_bytes,
                 csr_bytes=csr_bytes, density=density, diagonal_budget=budget, reason=reason)
+
+
+# ----------------------------------------------------------------------------------------------------------------------
+# tensordot
+# ----------------------------------------------------------------------------------------------------------------------
+def tensordot(A: np.ndarray, B: np.ndarray, axes) -> Dia:
+    """np.tensordot(A, B, axes) as the reference computes it (src/tensordotKernels.cpp:21-74): A's axes permuted to
+    (free, contracted in the order given) and read as a free_A x sum_A matrix of its nonzeros, B's to (contracted,
+    free) as sum_B x free_B (transpose, src/transposeKernels.cpp:19-110; matrix_reshape, src/reshapeKernels.cpp:21-73),
+    then product() with its zero-diagonal prune (src/spGEMM.cpp:516-534). A and B are dense tensors; returns the
+    free_A x free_B DIA result (its tensor shape is A's free axes then B's)."""
+    ax_a, ax_b = [list(a) for a in axes]
+    if len(ax_a) != len(ax_b):
+        raise ValueError("tensordot: axes must be two lists of the same length")
+    free_a = [i for i in range(A.ndim) if i not in ax_a]
+    free_b = [i for i in range(B.ndim) if i not in ax_b]
+    fa = int(np.prod([A.shape[i] for i in free_a])) if free_a else 1
+    fb = int(np.prod([B.shape[i] for i in free_b])) if free_b else 1
+    sa = int(np.prod([A.shape[i] for i in ax_a])) if ax_a else 1
+    Am = np.transpose(A, free_a + ax_a).reshape(fa, sa)
+    Bm = np.transpose(B, ax_b + free_b).reshape(sa, fb)
+    return prune(product(from_dense(Am), from_dense(Bm)))
 
 
 # ----------------------------------------------------------------------------------------------------------------------
@@ -601,6 +628,8 @@ PINS = {
                   "the spKron.cpp:2231-2262 prime-gate case against the kron matrix",
     "dia_viability": "hand-computed budget cases",
     "mlp_loss_grad": "central finite differences of mlp_forward's loss",
+    "tensordot": "np.tensordot values; offsets = the nonzero diagonals of the dense result "
+                 "(tests/test_oracle.py::test_tensordot_vs_numpy)",
     "adam_step": "closed forms worked by hand: step 1 from zero moments, step 2 with a zero gradient, a constant "
                  "gradient (tests/test_oracle.py::test_adam_step_closed_forms)",
 }

@@ -842,6 +842,17 @@ def test_spmv_ntimes_device_resident_and_in_place(d, n_times):
     if n_times == 0:
         with pytest.raises(RuntimeError, match="requires square"):
             d.spmv_ntimes_device(R, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 0)
+    else:  # no diagonals, rectangular: y = 0 for any count, no work vectors needed (src/spMV_ntimes_gpu.cu:98-105)
+        Z = d.to_device(d.from_numpy(np.zeros((3, 4), complex)))
+        assert Z.num_diags == 0
+        zr = torch.full((3,), 7.0, dtype=torch.float64, device="cuda")
+        zi = zr.clone()
+        d.spmv_ntimes_device(Z, xr.data_ptr(), xi.data_ptr(), zr.data_ptr(), zi.data_ptr(), n_times)
+        torch.cuda.synchronize()
+        assert not bool(torch.any(zr)) and not bool(torch.any(zi))
+        hz = d.to_numpy_vector(d.spMV_ntimes(d.from_numpy(np.zeros((3, 4), complex)), d.from_numpy_vector(x[:4]),
+                                             n_times))
+        np.testing.assert_array_equal(hz, np.zeros(3))
 
 
 @pytest.mark.parametrize("model", ["tfim", "xxz"])
@@ -917,10 +928,11 @@ def test_spmv_ntimes_zero_block_skip_is_exact(d, model):
         capi.check(L.dq_set_tuning(b"spmv_skip_zero_blocks", 3))
 
 
-def test_spmv_tuned_zero_block_map_is_exact_and_dropped_on_rewrite(d):
+def test_spmv_tuned_zero_block_map_is_exact_and_refreshed_on_rewrite(d):
     """dq_spmv_tune keeps the map of an FP64 matrix's all-zero 256-row blocks, and later SpMVs skip them: the same bits
-    as before tuning. Rewriting the values (dq_devmat_set_packed_device) drops the map, so the blocks that became
-    nonzero count again. An FP32 matrix keeps the same bits whether tuned or not (its SpMVs do not use the map)."""
+    as before tuning. Rewriting the values (dq_devmat_set_packed_device) recomputes the map for the new values, so the
+    blocks that became nonzero count again. An FP32 matrix keeps the same bits whether tuned or not (its SpMVs do not
+    use the map)."""
     import ctypes
     import torch
     from diaq_amd import capi
@@ -957,6 +969,63 @@ def test_spmv_tuned_zero_block_map_is_exact_and_dropped_on_rewrite(d):
             lens = [len(v) for v in vals]
             new_vals = np.split(packed, np.cumsum(lens)[:-1])
             assert rel(spmv(D, dt), O.spmv(O.Dia(rows, cols, offs, new_vals), x)) < TOL
+
+
+def test_captured_ntimes_after_value_rewrite_of_a_tuned_matrix(d):
+    """A graph captured around spMV_ntimes on a tuned TFIM reads the zero-block map kept in the handle. Rewriting the
+    values afterwards so that a zero block becomes nonzero recomputes that map in place, so replays of the old graph
+    match the eager run with the new values and the oracle."""
+    import torch
+    from diaq_amd import capi
+    L = capi.lib()
+    q = 12
+    rows, cols, offs, vals = Hm.tfim(q, 1.0, 0.7)
+    D = d.to_device(dia_of(d, rows, cols, offs, vals))
+    sid = torch.cuda.current_stream().cuda_stream
+    capi.check(L.dq_spmv_tune(D.handle, 2, sid, None))
+    rng = np.random.default_rng(80)
+    x = rng.standard_normal(rows) + 1j * rng.standard_normal(rows)
+    x /= np.linalg.norm(x)
+    xr = torch.tensor(x.real, device="cuda")
+    xi = torch.tensor(x.imag, device="cuda")
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    wr, wi = torch.empty_like(xr), torch.empty_like(xi)
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=cs):
+        d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 3, wr.data_ptr(),
+                             wi.data_ptr(), cs.cuda_stream)
+    graph.replay()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(yr.cpu().numpy() + 1j * yi.cpu().numpy(),
+                               O.spmv_ntimes(O.Dia(rows, cols, offs, vals), x, 3), rtol=0, atol=1e-12)
+    # the +2^8 diagonal holds -h on the rows whose bit 8 is 0, so its rows 256..511 form an all-zero 256-row block;
+    # give that block nonzero values and keep the rest
+    k = offs.index(1 << 8)
+    assert not np.any(vals[k][256:512])
+    new_vals = [np.array(v, copy=True) for v in vals]
+    new_vals[k][256:512] = 0.25 + 0.5j
+    packed = np.concatenate(new_vals)
+    re = torch.tensor(packed.real, device="cuda")
+    im = torch.tensor(packed.imag, device="cuda")
+    capi.check(L.dq_devmat_set_packed_device(D.handle, re.data_ptr(), im.data_ptr(), sid))
+    torch.cuda.synchronize()
+    ref = O.spmv_ntimes(O.Dia(rows, cols, offs, new_vals), x, 3)
+    out = {}
+    for tag in ("replay", "replay2"):
+        yr.fill_(7.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        out[tag] = yr.cpu().numpy() + 1j * yi.cpu().numpy()
+    d.spmv_ntimes_device(D, xr.data_ptr(), xi.data_ptr(), yr.data_ptr(), yi.data_ptr(), 3, wr.data_ptr(),
+                         wi.data_ptr(), sid)
+    torch.cuda.synchronize()
+    eager = yr.cpu().numpy() + 1j * yi.cpu().numpy()
+    assert rel(eager, ref) < TOL
+    np.testing.assert_array_equal(out["replay"], eager)
+    np.testing.assert_array_equal(out["replay2"], eager)
 
 
 def test_spmv_tuned_zero_block_map_rectangular_and_ragged(d):

@@ -511,3 +511,19 @@ def test_c_kron_apply_tasks_matches_numpy_oracle():
         C.kron_apply_tasks(b, (g, offs, starts, re, im), a, np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag),
                            yr, yi)
         np.testing.assert_allclose(yr + 1j * yi, O.kron_apply(b, M, a, x, y0), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("sa,sb,axes", [((4, 6), (6, 5), ([1], [0])), ((2, 3, 4), (4, 3, 5), ([1, 2], [1, 0])),
+                                        ((3, 4), (3, 4), ([0, 1], [0, 1])), ((2, 2, 2, 2), (2, 2), ([3], [1])),
+                                        ((5, 3), (2, 7), ([], []))])
+def test_tensordot_vs_numpy(sa, sb, axes):
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal(sa) + 1j * rng.standard_normal(sa)
+    B = rng.standard_normal(sb) + 1j * rng.standard_normal(sb)
+    A[rng.random(sa) < 0.4] = 0
+    B[rng.random(sb) < 0.4] = 0
+    want = np.tensordot(A, B, axes=axes)
+    got = O.tensordot(A, B, axes)
+    dense = want.reshape(got.rows, got.cols)
+    np.testing.assert_allclose(O.to_dense(got), dense, atol=1e-13)
+    assert got.offsets == O.from_dense(dense).offsets
