@@ -57,6 +57,8 @@ SIGNATURES = {
     "dq_spmv_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double)]),
     "dq_spmv_ntimes_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                       c_uintptr]),
+    "dq_spmv_ntimes_device_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                              c_void_p, c_uintptr]),
     "dq_spmv_ntimes_host": (c_int, [c_void_p, P(c_double), P(c_double), P(c_double), P(c_double), c_int]),
     "dq_spmv_device_banded": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_uintptr]),
     "dq_release_cached_memory": (c_int, []),

@@ -23,6 +23,7 @@
 // FP32: values and vectors are float; each row's diagonal sum is Kahan-compensated in FP32.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 #include "dia_rows.hpp"
 
@@ -583,6 +584,38 @@ static void ntimes_device(const dq_devmat *A, const void *xr, const void *xi, vo
 extern "C" int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int n,
                                      void *work_r, void *work_i, uintptr_t stream) {
     return guard([&] { ntimes_device(A, xr, xi, yr, yi, n, work_r, work_i, as_stream(stream), nullptr, 0); });
+}
+
+// The allocation-free form's scratch: room for the zero-block map of a sequence (256-row blocks, one bit each, for
+// every diagonal), 16 MiB per GPU, held from the library's load until the process ends. In host code `scratch` only
+// stands in for the symbol; its device address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t NOALLOC_SCRATCH_WORDS = int64_t(1) << 22;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static uint32_t *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// This device's copy of `scratch`, looked up on the first call on the device (with that device current).
+static uint32_t *noalloc_scratch() {
+    int dev = 0;
+    DQ_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw ArgError("spmv_ntimes_device_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        DQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<uint32_t *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+extern "C" int dq_spmv_ntimes_device_noalloc(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi,
+                                             int n, void *work_r, void *work_i, uintptr_t stream) {
+    return guard([&] {
+        ntimes_device(A, xr, xi, yr, yi, n, work_r, work_i, as_stream(stream), noalloc_scratch(),
+                      NOALLOC_SCRATCH_WORDS);
+    });
 }
 
 extern "C" int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, double *yr, double *yi,

@@ -138,6 +138,17 @@ int dq_spmv_ntimes_host(const dq_devmat *A, const double *xr, const double *xi, 
 int dq_spmv_ntimes_device(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi, int num_times,
                           void *work_r, void *work_i, uintptr_t stream);
 
+/* dq_spmv_ntimes_device without any device allocation: the same products, results and errors, but the zero-block map
+ * a sequence of two or more products makes for an untuned matrix is written to a static 16 MiB device array of the
+ * library (one per GPU, held until the process ends) instead of stream-ordered memory, and it is made even while
+ * `stream` is being captured into a graph. A map that would not fit (more than 2^22 words: diagonals times
+ * ceil(rows / 8192)) is an error; a matrix tuned by dq_spmv_tune uses the map kept in its handle and needs no scratch.
+ * All calls on one GPU share the array: calls that overlap on one GPU (other streams, other threads) race, and a
+ * captured graph that made its map there must not be replayed while another such call runs. No reference
+ * counterpart: for callers that cannot allocate device memory (code under graph capture, serving loops). */
+int dq_spmv_ntimes_device_noalloc(const dq_devmat *A, const void *xr, const void *xi, void *yr, void *yi,
+                                  int num_times, void *work_r, void *work_i, uintptr_t stream);
+
 /* y = A x with the column range split into n_bands passes (n_bands = 0 picks the library's choice for the size);
  * same result as dq_spmv_device up to summation order. No reference counterpart (tuning entry point). */
 int dq_spmv_device_banded(const dq_devmat *A, const void *xr_dev, const void *xi_dev, void *yr_dev, void *yi_dev,
