@@ -87,35 +87,51 @@ void blas_check(rocblas_status st, const char *what) {
         throw HipError(std::string(what) + " failed: " + rocblas_status_to_string(st));
 }
 
-// Result offsets of A B: every in-range dA + dB, ascending (the structural dry-run without its pair lists), from a
-// bitset of B's offsets shifted by each of A's.
+// Result offsets of A B: every in-range dA + dB, ascending (the structural dry-run without its pair lists). The
+// result is a bitset over [lo, hi]; each of A's offsets ORs in B's offset bitset shifted by dA, a whole word at a
+// time (nA x nB / 64 word operations: 8191 x 8191 offsets take ~1 M).
 std::vector<int32_t> sum_offsets(const dq_devmat &A, const dq_devmat &B) {
     const int64_t lo = -static_cast<int64_t>(A.rows) + 1, hi = static_cast<int64_t>(B.cols) - 1;
     const int64_t range = hi - lo + 1;
     std::vector<int32_t> out;
-    if (range <= 0) return out;
-    // B's offsets as bits over [lo_b, lo_b + span)
+    if (range <= 0 || A.n() == 0 || B.n() == 0) return out;
+    // B's offsets as bits over [lo_b, lo_b + 64 * nbw)
     int64_t lo_b = INT64_MAX, hi_b = INT64_MIN;
     for (int32_t d : B.offs) { lo_b = std::min<int64_t>(lo_b, d); hi_b = std::max<int64_t>(hi_b, d); }
-    const int64_t words = (range + 63) / 64;
-    std::vector<uint64_t> res(static_cast<size_t>(words), 0);
-    std::vector<uint64_t> bb(static_cast<size_t>((hi_b - lo_b + 1 + 63) / 64 + 1), 0);
+    const int64_t nbw = (hi_b - lo_b + 1 + 63) / 64;
+    std::vector<uint64_t> bb(static_cast<size_t>(nbw), 0);
     for (int32_t d : B.offs) bb[(d - lo_b) / 64] |= uint64_t(1) << ((d - lo_b) % 64);
+    // result bits, offset by `base` bits so that every shifted word lands inside the array: bit i of bb (offset
+    // lo_b + i) goes to result bit da + lo_b + i - lo, which may fall outside [0, range) for out-of-range sums
+    int64_t pmin = INT64_MAX, pmax = INT64_MIN;
     for (int32_t da : A.offs) {
-        // bit i of bb means offset lo_b + i; sum = da + lo_b + i lands at result bit da + lo_b + i - lo
-        const int64_t shift = static_cast<int64_t>(da) + lo_b - lo;
-        for (size_t w = 0; w < bb.size(); ++w) {
-            const uint64_t v = bb[w];
-            if (!v) continue;
-            const int64_t bit0 = shift + static_cast<int64_t>(w) * 64;  // result bit of v's bit 0
-            for (uint64_t x = v; x; x &= x - 1) {
-                const int64_t bit = bit0 + __builtin_ctzll(x);
-                if (bit >= 0 && bit < range) res[bit / 64] |= uint64_t(1) << (bit % 64);
+        pmin = std::min<int64_t>(pmin, static_cast<int64_t>(da) + lo_b - lo);
+        pmax = std::max<int64_t>(pmax, static_cast<int64_t>(da) + lo_b - lo);
+    }
+    const int64_t base = pmin < 0 ? ((-pmin + 63) / 64) * 64 : 0;
+    const int64_t words = (range + 63) / 64;
+    const int64_t total = std::max<int64_t>(base / 64 + words, (pmax + base) / 64 + nbw + 1) + 1;
+    std::vector<uint64_t> res(static_cast<size_t>(total), 0);
+    for (int32_t da : A.offs) {
+        const int64_t shift = static_cast<int64_t>(da) + lo_b - lo + base;  // >= 0
+        const int64_t w0 = shift / 64;
+        const int sh = static_cast<int>(shift % 64);
+        uint64_t *r = res.data() + w0;  // r[0 .. nbw] stay below total
+        if (sh == 0) {
+            for (int64_t w = 0; w < nbw; ++w) r[w] |= bb[w];
+        } else {
+            for (int64_t w = 0; w < nbw; ++w) {
+                r[w] |= bb[w] << sh;
+                r[w + 1] |= bb[w] >> (64 - sh);
             }
         }
     }
-    for (int64_t w = 0; w < words; ++w)
-        for (uint64_t x = res[w]; x; x &= x - 1) out.push_back(static_cast<int32_t>(lo + w * 64 + __builtin_ctzll(x)));
+    // result bit j (offset lo + j) sits at bit base + j; keep j in [0, range)
+    for (int64_t w = 0; w < words; ++w) {
+        uint64_t x = res[static_cast<size_t>(w + base / 64)];
+        if (w == words - 1 && (range % 64)) x &= (uint64_t(1) << (range % 64)) - 1;
+        for (; x; x &= x - 1) out.push_back(static_cast<int32_t>(lo + w * 64 + __builtin_ctzll(x)));
+    }
     return out;
 }
 }  // namespace

@@ -340,6 +340,35 @@ void scale_real(dq_devmat &M, double f, hipStream_t s) {
     DQ_LAUNCH_CHECK();
 }
 
+// A += B for two matrices with one layout (same shape, precision and offsets in the same order): one flat pass over
+// the value arrays, padding included (zero plus zero). The same sums as union_add's A + 1.0 * B.
+template <typename T>
+__global__ void add_flat_kernel(int64_t n, T *__restrict__ ar, T *__restrict__ ai, const T *__restrict__ br,
+                                const T *__restrict__ bi) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        ar[i] = ar[i] + br[i];
+        ai[i] = ai[i] + bi[i];
+    }
+}
+
+bool same_layout(const dq_devmat &A, const dq_devmat &B) {
+    return A.rows == B.rows && A.cols == B.cols && A.precision == B.precision && A.total == B.total &&
+           A.offs == B.offs;
+}
+
+void add_in_place(dq_devmat &A, const dq_devmat &B, hipStream_t s) {
+    if (!same_layout(A, B)) throw ArgError("add_in_place: layouts differ");
+    if (!A.total) return;
+    const unsigned g = blocks_for(A.total, 256, 8192);
+    if (A.precision == DQ_F64)
+        hipLaunchKernelGGL(add_flat_kernel<double>, dim3(g), dim3(256), 0, s, A.total, A.re.as<double>(),
+                           A.im.as<double>(), B.re.as<double>(), B.im.as<double>());
+    else
+        hipLaunchKernelGGL(add_flat_kernel<float>, dim3(g), dim3(256), 0, s, A.total, A.re.as<float>(),
+                           A.im.as<float>(), B.re.as<float>(), B.im.as<float>());
+    DQ_LAUNCH_CHECK();
+}
+
 // C = (i * imag) A: (re, im) -> (-1.0 * im * imag, re * imag) (src/diaQMatrix.hpp:1181-1211)
 template <typename T>
 __global__ void times_i_kernel(int64_t n, const T *__restrict__ ar, const T *__restrict__ ai, T *__restrict__ cr,

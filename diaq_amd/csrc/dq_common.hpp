@@ -193,6 +193,9 @@ void scale_real(dq_devmat &M, double factor, hipStream_t s);
 void times_i_scale_copy(const dq_devmat &A, dq_devmat &C, double imag, hipStream_t s);  // C = (i*imag) A
 // A + B over the ascending union of their diagonals (src/diaQMatrix.hpp:867-916); shapes must match.
 std::unique_ptr<dq_devmat> union_add(const dq_devmat &A, const dq_devmat &B, hipStream_t s);
+// Whether A and B share one layout (shape, precision, offsets in the same order), and A += B for such a pair.
+bool same_layout(const dq_devmat &A, const dq_devmat &B);
+void add_in_place(dq_devmat &A, const dq_devmat &B, hipStream_t s);
 double frobenius(const dq_devmat &A, hipStream_t s);
 double estimate_1norm(const dq_devmat &A, hipStream_t s);
 std::unique_ptr<dq_devmat> prune(const dq_devmat &A, hipStream_t s);

@@ -70,7 +70,11 @@ static std::unique_ptr<dq_devmat> taylor(const dq_devmat &X, int max_iterations,
     int i;
     for (i = 1; i <= max_iterations; ++i) {
         term = product(*term, X, static_cast<double>(i), s, true);
-        result = union_add(*result, *term, s);
+        // once the terms reach every diagonal the sum holds, the union is the sum's own layout: add in place
+        if (same_layout(*result, *term))
+            add_in_place(*result, *term, s);
+        else
+            result = union_add(*result, *term, s);
         if (frobenius(*term, s) < tol) break;
     }
     if (i == max_iterations) {  // the reference's check (one short of the loop bound), kept as is
