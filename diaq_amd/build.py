@@ -43,7 +43,7 @@ def build_hip(jobs: int = 8) -> str:
         list(ex.map(_run, [[HIPCC, *HIP_FLAGS, "-c", s, "-o", o] for s, o in zip(srcs, objs)]))
     lib = os.path.join(HERE, "libdiaq_hip.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, lib)
     return lib
 

@@ -408,6 +408,35 @@ void run_tiles(bool to_dia, const dq_devmat &M, const double *src, double *dst, 
     DQ_HIP(hipStreamSynchronize(s));  // the tables are freed on return
 }
 
+void dia_to_dense(const dq_devmat &M, double *dense, int64_t ld, hipStream_t s) {
+    const int64_t rows = M.rows, cols = M.cols;
+    if (rows == 0 || cols == 0) return;
+    if (sparse_dense_path(M)) {
+        hipLaunchKernelGGL(zero_dense_kernel, dim3(blocks_for(rows * cols, 256, 1 << 16)), dim3(256), 0, s, rows,
+                           cols, dense, ld);
+        DQ_LAUNCH_CHECK();
+        if (M.n()) {
+            const dim3 grid(blocks_for(rows, 256, 1LL << 30), (M.n() + RG_DIAGS - 1) / RG_DIAGS);
+            if (M.precision == DQ_F64)
+                hipLaunchKernelGGL(dia_to_dense_rows_kernel<double>, grid, dim3(256), 0, s, rows, M.n(),
+                                   M.d_meta.as<DiagMeta>(), M.re.as<double>(), M.im.as<double>(), dense, ld);
+            else
+                hipLaunchKernelGGL(dia_to_dense_rows_kernel<float>, grid, dim3(256), 0, s, rows, M.n(),
+                                   M.d_meta.as<DiagMeta>(), M.re.as<float>(), M.im.as<float>(), dense, ld);
+            DQ_LAUNCH_CHECK();
+        }
+        return;
+    }
+    if (M.precision == DQ_F64) run_tiles<double>(false, M, nullptr, dense, ld, s);
+    else run_tiles<float>(false, M, nullptr, dense, ld, s);
+}
+
+void dense_to_dia(dq_devmat &M, const double *dense, int64_t ld, hipStream_t s) {
+    if (M.rows == 0 || M.cols == 0 || M.n() == 0) return;
+    if (M.precision == DQ_F64) run_tiles<double>(true, M, dense, nullptr, ld, s);
+    else run_tiles<float>(true, M, dense, nullptr, ld, s);
+}
+
 }  // namespace dq
 
 using namespace dq;

@@ -1,17 +1,15 @@
 // Dense path for near-dense DIA products: the squarings of a matrix exponential whose result fills (nearly) every
 // diagonal, where diagonal-pair enumeration degenerates into a dense matrix product done one pair at a time.
 //
-// A and B are scattered into dense row-major complex matrices, multiplied by the library GEMM on the matrix cores
-// (rocBLAS zgemm / cgemm: FP64 / FP32 MFMA), and C's diagonals -- exactly the structural dry-run's offsets, every
-// in-range dA + dB -- are gathered back, divided by the divisor. The values agree with the pair-by-pair sums to
-// rounding (a different summation order); the offsets are the same. Used only inside pow and the matrix
-// exponential (where no prune follows), never by the user-facing product, whose pair order matches the reference's.
-#include <rocblas/rocblas.h>
+// A and B are expanded into dense row-major complex double matrices (the tiled conversion kernels of builders.hip),
+// multiplied by the complex FP64 MFMA GEMM of gemm.hip (which also divides by the divisor), and C's diagonals --
+// exactly the structural dry-run's offsets, every in-range dA + dB -- are gathered back. An FP32 matrix is widened
+// to FP64 for the product and rounded once at the end. The values agree with the pair-by-pair sums to rounding (a
+// different summation order); the offsets are the same. Used only inside pow and the matrix exponential (where no
+// prune follows), never by the user-facing product, whose pair order matches the reference's.
 
 #include <algorithm>
-#include <map>
-#include <mutex>
-#include <type_traits>
+#include <climits>
 
 #include "dq_common.hpp"
 
@@ -36,57 +34,7 @@ bool dense_product_pays(const dq_devmat &A, const dq_devmat &B) {
     return pair_work >= 0.25 * static_cast<double>(M) * K * N;
 }
 
-template <typename T>
-__global__ void dia_to_dense_kernel(const DiagMeta *__restrict__ meta, const T *__restrict__ re,
-                                    const T *__restrict__ im, T *__restrict__ D, int64_t ld) {
-    const DiagMeta m = meta[blockIdx.y];
-    const int64_t len = m.rhi - m.rlo;
-    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < len;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t r = m.rlo + p, c = r + m.off;
-        D[2 * (r * ld + c)] = re[m.rowbase + r];
-        D[2 * (r * ld + c) + 1] = im[m.rowbase + r];
-    }
-}
-
-template <typename T>
-__global__ void dense_to_dia_kernel(const DiagMeta *__restrict__ meta, const T *__restrict__ D, int64_t ld,
-                                    T *__restrict__ re, T *__restrict__ im, double divisor) {
-    const DiagMeta m = meta[blockIdx.y];
-    const int64_t len = m.rhi - m.rlo;
-    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < len;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t r = m.rlo + p, c = r + m.off;
-        T vr = D[2 * (r * ld + c)], vi = D[2 * (r * ld + c) + 1];
-        if (divisor != 1.0) {
-            vr = static_cast<T>(vr / divisor);
-            vi = static_cast<T>(vi / divisor);
-        }
-        re[m.rowbase + r] = vr;
-        im[m.rowbase + r] = vi;
-    }
-}
-
 namespace {
-rocblas_handle blas_handle() {
-    static std::mutex mu;
-    static std::map<int, rocblas_handle> *handles = new std::map<int, rocblas_handle>;  // per device, kept
-    int dev = 0;
-    DQ_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = handles->find(dev);
-    if (it != handles->end()) return it->second;
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) throw HipError("rocblas_create_handle failed");
-    (*handles)[dev] = h;
-    return h;
-}
-
-void blas_check(rocblas_status st, const char *what) {
-    if (st != rocblas_status_success)
-        throw HipError(std::string(what) + " failed: " + rocblas_status_to_string(st));
-}
-
 // Result offsets of A B: every in-range dA + dB, ascending (the structural dry-run without its pair lists). The
 // result is a bitset over [lo, hi]; each of A's offsets ORs in B's offset bitset shifted by dA, a whole word at a
 // time (nA x nB / 64 word operations: 8191 x 8191 offsets take ~1 M).
@@ -140,51 +88,23 @@ std::unique_ptr<dq_devmat> product_dense(const dq_devmat &A, const dq_devmat &B,
     if (A.cols != B.rows) throw ArgError("diaQMatrix product_cuda: dimension mismatch.");
     if (A.precision != B.precision) throw ArgError("product: operands have different precisions");
     const std::vector<int32_t> res = sum_offsets(A, B);
-    auto C = make_devmat(A.rows, B.cols, res.data(), static_cast<int>(res.size()), A.precision, s);  // zeroed
+    // every stored element of C is written from the dense product; only the gaps between diagonals are zeroed
+    auto C = make_devmat(A.rows, B.cols, res.data(), static_cast<int>(res.size()), A.precision, s, false);
     if (res.empty()) return C;
+    zero_gaps(*C, s);
     const int64_t M = A.rows, K = A.cols, N = B.cols;
-    const size_t eb = 2 * static_cast<size_t>(elem_bytes(A.precision));  // complex element
-    DevBuf Ad(static_cast<size_t>(M * K) * eb), Bd(static_cast<size_t>(K * N) * eb), Cd(static_cast<size_t>(M * N) * eb);
-    zero_bytes(Ad.p, Ad.bytes, s);
-    zero_bytes(Bd.p, Bd.bytes, s);
-    rocblas_handle h = blas_handle();
-    blas_check(rocblas_set_stream(h, s), "rocblas_set_stream");
-    auto run = [&](auto tag) {
-        using T = decltype(tag);
-        if (A.n())
-            hipLaunchKernelGGL(dia_to_dense_kernel<T>, dim3(std::max<unsigned>(1, blocks_for(std::min(M, K), 256, 64)),
-                               static_cast<unsigned>(A.n())), dim3(256), 0, s, A.d_meta.as<DiagMeta>(), A.re.as<T>(),
-                               A.im.as<T>(), Ad.as<T>(), K);
-        if (B.n())
-            hipLaunchKernelGGL(dia_to_dense_kernel<T>, dim3(std::max<unsigned>(1, blocks_for(std::min(K, N), 256, 64)),
-                               static_cast<unsigned>(B.n())), dim3(256), 0, s, B.d_meta.as<DiagMeta>(), B.re.as<T>(),
-                               B.im.as<T>(), Bd.as<T>(), N);
-        DQ_LAUNCH_CHECK();
-        // row-major C = A B is column-major C^T = B^T A^T
-        if constexpr (std::is_same<T, double>::value) {
-            const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-            blas_check(rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, static_cast<rocblas_int>(N),
-                                     static_cast<rocblas_int>(M), static_cast<rocblas_int>(K), &one,
-                                     Bd.as<rocblas_double_complex>(), static_cast<rocblas_int>(N),
-                                     Ad.as<rocblas_double_complex>(), static_cast<rocblas_int>(K), &zero,
-                                     Cd.as<rocblas_double_complex>(), static_cast<rocblas_int>(N)),
-                       "rocblas_zgemm");
-        } else {
-            const rocblas_float_complex one = {1.0f, 0.0f}, zero = {0.0f, 0.0f};
-            blas_check(rocblas_cgemm(h, rocblas_operation_none, rocblas_operation_none, static_cast<rocblas_int>(N),
-                                     static_cast<rocblas_int>(M), static_cast<rocblas_int>(K), &one,
-                                     Bd.as<rocblas_float_complex>(), static_cast<rocblas_int>(N),
-                                     Ad.as<rocblas_float_complex>(), static_cast<rocblas_int>(K), &zero,
-                                     Cd.as<rocblas_float_complex>(), static_cast<rocblas_int>(N)),
-                       "rocblas_cgemm");
-        }
-        hipLaunchKernelGGL(dense_to_dia_kernel<T>, dim3(std::max<unsigned>(1, blocks_for(std::min(M, N), 256, 64)),
-                           static_cast<unsigned>(C->n())), dim3(256), 0, s, C->d_meta.as<DiagMeta>(), Cd.as<T>(), N,
-                           C->re.as<T>(), C->im.as<T>(), divisor);
-        DQ_LAUNCH_CHECK();
-    };
-    if (A.precision == DQ_F64) run(double{});
-    else run(float{});
+    const size_t cb = 2 * sizeof(double);  // an interleaved complex double
+    DevBuf Ad(static_cast<size_t>(M * K) * cb), Cd(static_cast<size_t>(M * N) * cb);
+    DevBuf Bd;
+    dia_to_dense(A, Ad.as<double>(), K, s);
+    const double *bd = Ad.as<double>();
+    if (&A != &B) {  // a squaring reads one dense copy twice
+        Bd = DevBuf(static_cast<size_t>(K * N) * cb);
+        dia_to_dense(B, Bd.as<double>(), N, s);
+        bd = Bd.as<double>();
+    }
+    zgemm_dense(M, N, K, Ad.as<double>(), bd, Cd.as<double>(), divisor, s);
+    dense_to_dia(*C, Cd.as<double>(), N, s);
     DQ_HIP(hipStreamSynchronize(s));  // before the dense buffers go back to the cache
     return C;
 }

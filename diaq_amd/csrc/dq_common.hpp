@@ -215,9 +215,16 @@ bool evo_set_tuning(const std::string &key, int value);
 // path (dense.hip) -- same offsets, values equal up to the summation order; for pow and the matrix exponential.
 std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s,
                                    bool allow_dense = false);
-// Dense path (dense.hip): scatter to dense, library GEMM on the matrix cores, gather C's diagonals.
+// Dense path (dense.hip): expand to dense, the complex GEMM of gemm.hip, gather C's diagonals.
 std::unique_ptr<dq_devmat> product_dense(const dq_devmat &A, const dq_devmat &B, double divisor, hipStream_t s);
 bool dense_product_pays(const dq_devmat &A, const dq_devmat &B);
+// Dense side of the dense path (builders.hip): every element of the rows x cols interleaved complex double array
+// (leading dimension ld) from M's diagonals (zeros elsewhere), and M's stored elements back from such an array.
+void dia_to_dense(const dq_devmat &M, double *dense, int64_t ld, hipStream_t s);
+void dense_to_dia(dq_devmat &M, const double *dense, int64_t ld, hipStream_t s);
+// C = (A B) / divisor for interleaved complex double row-major dense matrices, on the matrix cores (gemm.hip).
+void zgemm_dense(int64_t M, int64_t N, int64_t K, const double *A, const double *B, double *C, double divisor,
+                 hipStream_t s);
 // Dense-path tuning key ("product_dense"); false if it is another key.
 bool dense_set_tuning(const std::string &key, int value);
 // Dense <-> DIA conversion tuning key ("dense_convert"); false if it is another key.

@@ -657,8 +657,9 @@ def test_device_block_cache_reuse(d):
 
 
 def test_dense_path_matches_pair_products(d):
-    """Near-dense products inside pow and expm take the dense path (library GEMM on the matrix cores): the offsets are
-    those of the pair-by-pair products and the values agree to rounding, against the pair path and the oracle."""
+    """Near-dense products inside pow and expm take the dense path (the library's complex MFMA GEMM): the offsets are
+    those of the pair-by-pair products and the values agree to rounding, against the pair path and the oracle, for a
+    size that is not a multiple of the GEMM tile (300) and in FP32."""
     from diaq_amd import capi
     L = capi.lib()
     rng = np.random.default_rng(21)
@@ -676,6 +677,13 @@ def test_dense_path_matches_pair_products(d):
             assert rel(outs[dense][1], ref) < TOL, dense
         assert outs[0][0] == outs[1][0]
         np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=0, atol=1e-13 * np.abs(ref).max())
+        # FP32 operands: widened to FP64 for the dense product, rounded once at the end
+        D32 = d.to_device(H, precision="fp32")
+        for dense in (0, 1):
+            capi.check(L.dq_set_tuning(b"product_dense", dense))
+            P32 = d.pow(D32, 4)
+            assert list(P32.diagonals) == outs[0][0]
+            assert rel(d.to_numpy(P32.to_host()), ref) < 2e-6, dense
         # expm of a 10-qubit TFIM: the final squarings are near-dense
         rows, cols, to, tv = Hm.tfim(10, 1.0, 0.7)
         Ud, roffs, _, s = O.expm_negative_dense(O.to_dense(O.Dia(rows, cols, to, tv)), to, 0.4)
