@@ -822,7 +822,18 @@ def expm_section(args, torch, d, capi, Hm):
     UD = d.expm_negative(D, 0.1)
     torch.cuda.synchronize()
     t_dev = time.perf_counter() - t0
-    del UD
+    # the near-dense squaring alone: U (8191 diagonals = a dense 4096 x 4096 complex matrix) times itself through the
+    # dense path (DIA -> dense, the complex FP64 MFMA GEMM, dense -> DIA); 8 N^3 flops
+    d.pow(UD, 2)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P2 = d.pow(UD, 2)
+    torch.cuda.synchronize()
+    t_sq = (time.perf_counter() - t0) / reps
+    del UD, P2
+    sq_flops = 8.0 * r ** 3
     dense = np.zeros((r, c), complex)
     for o, v in zip(offs, vals):
         k = np.arange(len(v))
@@ -833,8 +844,12 @@ def expm_section(args, torch, d, capi, Hm):
     err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
     log(f"[bench] expm_negative TFIM-12 t=0.1: {t:.3f} s, {U.num_diags()} diagonals, max rel error vs scipy {err:.1e}; "
         f"on a DeviceMatrix (result left on the device) {t_dev:.3f} s")
+    log(f"[bench] dense squaring of the 4096 x 4096 propagator (pow(U, 2) on the device): {t_sq * 1e3:.2f} ms, "
+        f"{sq_flops / t_sq / 1e12:.1f} TFLOP/s including the conversions")
     return dict(workload="expm(-i H t), TFIM-12 (J=1, h=0.7), t=0.1, defaults max_it 100, cutoff 1, tol 1e-12",
-                seconds=t, n_diagonals=U.num_diags(), max_rel_err_vs_scipy_expm=err, seconds_device_matrix=t_dev)
+                seconds=t, n_diagonals=U.num_diags(), max_rel_err_vs_scipy_expm=err, seconds_device_matrix=t_dev,
+                dense_squaring=dict(seconds=t_sq, flops=sq_flops, tflops_incl_conversions=sq_flops / t_sq / 1e12,
+                                    peak_fp64_mfma_tflops=78.6))
 
 
 def batched_section(args, torch, d, capi, Hm, quick=False):
