@@ -162,6 +162,129 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
     }
 }
 
+// The same Horner level with four states per lane (NS = 4, 8 or 16: NS / 4 lanes per row): each diagonal's gather is
+// one 16-byte load of 4 adjacent states' real parts and one of their imaginary parts, and a lane's matrix value serves
+// its 4 states, so a wave issues a quarter of the vector memory instructions of horner_kernel for the same bytes. A
+// wave covers 256 / NS rows per pass -- exactly one block of the zero-block map -- and the workgroup RPT passes of
+// 1024 / NS rows. Each (row, state) sums its diagonals in the same order with the same operations as horner_kernel,
+// so the two kernels give the same bits.
+template <typename TV, int NS, int RPT>
+__global__ __launch_bounds__(256) void horner4_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
+                                                      int64_t total, const TV *__restrict__ hre,
+                                                      const TV *__restrict__ him, const JobParam *__restrict__ params,
+                                                      int64_t g, double *__restrict__ p64r, double *__restrict__ p64i,
+                                                      float *__restrict__ p32r, float *__restrict__ p32i,
+                                                      const float *__restrict__ bin_r, const float *__restrict__ bin_i,
+                                                      float *__restrict__ bout_r, float *__restrict__ bout_i,
+                                                      int64_t ntiles, int xcd, const uint32_t *__restrict__ zmask,
+                                                      int64_t zwords) {
+    static_assert(NS == 4 || NS == 8 || NS == 16, "four states per lane");
+    constexpr int L = NS / 4;         // lanes per row
+    constexpr int WR = 64 / L;        // rows per wave and pass: one zero-map block (256 / NS rows)
+    constexpr int RPP = 4 * WR;       // rows per workgroup pass
+    constexpr int T = RPP * RPT;      // rows per tile: 4 RPT blocks, inside one 32-bit word of the map
+    const int j = blockIdx.y;
+    const JobParam P = params[j];
+    if (g >= P.steps) return;
+    const int64_t t = tile_of_block(blockIdx.x, ntiles, xcd);
+    if (t >= ntiles) return;  // uniform: a surplus workgroup of the XCD-contiguous grid
+    const int m = P.m;
+    const int k = m - 1 - static_cast<int>(g % m);  // Horner level
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane % L;                          // states 4h .. 4h + 3 of the row
+    const int rw = lane / L;
+    const int64_t tile = t * T;
+    const int64_t vb = static_cast<int64_t>(j) * total;
+    const int64_t sb = static_cast<int64_t>(j) * N * NS;
+    const float *xr = bin_r + sb;
+    const float *xi = bin_i + sb;
+    const int64_t blk0 = t * 4 * RPT;  // the tile's first map block
+    Acc<TV> hr[RPT][4], hi[RPT][4];
+    for (int d = 0; d < n; ++d) {
+        const DiagMeta md = meta[d];
+        const int64_t lo = md.rlo, hi_ = md.rhi;
+        if (hi_ <= tile || lo >= tile + T) continue;  // uniform
+        uint32_t live = (4 * RPT >= 32) ? ~0u : ((1u << (4 * RPT)) - 1);
+        if (zmask) {
+            const uint32_t w = zmask[(static_cast<int64_t>(j) * n + d) * zwords + (blk0 >> 5)];
+            live = (4 * RPT >= 32) ? w : ((w >> (blk0 & 31)) & ((1u << (4 * RPT)) - 1));
+            if (!live) continue;  // uniform: every value of the diagonal in this tile is zero
+        }
+        const TV *ar = hre + vb + md.rowbase;
+        const TV *ai = him + vb + md.rowbase;
+        TV va_r[RPT] = {}, va_i[RPT] = {};
+        float4 vx_r[RPT], vx_i[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            vx_r[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+            vx_i[q] = vx_r[q];
+            if (!((live >> (4 * q + wave)) & 1u)) continue;  // uniform in the wave
+            const int64_t r = tile + RPP * q + WR * wave + rw;
+            const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);  // stays inside the tile
+            va_r[q] = ar[rc];
+            va_i[q] = ai[rc];
+            const int64_t xo = (rc + md.off) * NS + 4 * h;
+            vx_r[q] = *reinterpret_cast<const float4 *>(xr + xo);
+            vx_i[q] = *reinterpret_cast<const float4 *>(xi + xo);
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
+            const int64_t r = tile + RPP * q + WR * wave + rw;
+            if (((live >> (4 * q + wave)) & 1u) && r >= lo && r < hi_) {
+                cmac(hr[q][0], hi[q][0], va_r[q], va_i[q], static_cast<TV>(vx_r[q].x), static_cast<TV>(vx_i[q].x));
+                cmac(hr[q][1], hi[q][1], va_r[q], va_i[q], static_cast<TV>(vx_r[q].y), static_cast<TV>(vx_i[q].y));
+                cmac(hr[q][2], hi[q][2], va_r[q], va_i[q], static_cast<TV>(vx_r[q].z), static_cast<TV>(vx_i[q].z));
+                cmac(hr[q][3], hi[q][3], va_r[q], va_i[q], static_cast<TV>(vx_r[q].w), static_cast<TV>(vx_i[q].w));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = tile + RPP * q + WR * wave + rw;
+        if (r >= N) continue;
+        const int64_t o = sb + r * NS + 4 * h;
+        double a[4], b[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            a[v] = static_cast<double>(hr[q][v].get());
+            b[v] = static_cast<double>(hi[q][v].get());
+        }
+        if (k > 0) {  // b_k = psi + (Y/(k+1)) b_{k+1}, (Y/c) h = (tau/c) (Im h, -Re h)
+            const double c = P.tau / static_cast<double>(k + 1);
+            const float4 pr = *reinterpret_cast<const float4 *>(p32r + o);
+            const float4 pi = *reinterpret_cast<const float4 *>(p32i + o);
+            float4 outr, outi;
+            outr.x = static_cast<float>(static_cast<double>(pr.x) + c * b[0]);
+            outr.y = static_cast<float>(static_cast<double>(pr.y) + c * b[1]);
+            outr.z = static_cast<float>(static_cast<double>(pr.z) + c * b[2]);
+            outr.w = static_cast<float>(static_cast<double>(pr.w) + c * b[3]);
+            outi.x = static_cast<float>(static_cast<double>(pi.x) - c * a[0]);
+            outi.y = static_cast<float>(static_cast<double>(pi.y) - c * a[1]);
+            outi.z = static_cast<float>(static_cast<double>(pi.z) - c * a[2]);
+            outi.w = static_cast<float>(static_cast<double>(pi.w) - c * a[3]);
+            *reinterpret_cast<float4 *>(bout_r + o) = outr;
+            *reinterpret_cast<float4 *>(bout_i + o) = outi;
+        } else {  // psi <- psi + Y b_1 in FP64; its FP32 copy goes to p32 (own entries) and to the next level's input
+            float nr32[4], ni32[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double nr = p64r[o + v] + P.tau * b[v];
+                const double ni = p64i[o + v] - P.tau * a[v];
+                p64r[o + v] = nr;
+                p64i[o + v] = ni;
+                nr32[v] = static_cast<float>(nr);
+                ni32[v] = static_cast<float>(ni);
+            }
+            const float4 fr = make_float4(nr32[0], nr32[1], nr32[2], nr32[3]);
+            const float4 fi = make_float4(ni32[0], ni32[1], ni32[2], ni32[3]);
+            *reinterpret_cast<float4 *>(p32r + o) = fr;
+            *reinterpret_cast<float4 *>(p32i + o) = fi;
+            *reinterpret_cast<float4 *>(bout_r + o) = fr;
+            *reinterpret_cast<float4 *>(bout_i + o) = fi;
+        }
+    }
+}
+
 // Forward Taylor terms of the first sub-step for the a-posteriori estimate: t_k = (Y/k) t_{k-1}, t_0 = psi (FP32);
 // the squared norms of t_m and t_{m+1} of every state are accumulated into norms[(j*NS + state)*2 + (k - m)].
 template <typename TV, int NS, int RPT>
@@ -322,6 +445,7 @@ struct EvoClass {
     int rpt = 4;              // tuned rows per lane
     int xcd = 0;              // tuned row-tile order: 0 round-robin over the XCDs, 1 XCD-contiguous
     int lds = 0;              // tuned: 1 = stage each tile's input window in LDS
+    int vec = 0;              // tuned: 1 = horner4_kernel (four states per lane; NS = 4, 8, 16, no LDS window)
     hipStream_t stream = nullptr;
     int jobs_n() const { return static_cast<int>(jobs.size()); }
 };
@@ -353,6 +477,7 @@ static int g_evo_rpt = 0;   // 0 = the class's, else 1, 2, 4 or 8
 static int g_evo_xcd = -1;  // -1 = the class's, else 0 or 1
 static int g_evo_lds = -1;  // -1 = the class's, else 0 or 1
 static int g_evo_skip = 1;  // skip the all-zero blocks of a diagonal ("evo_skip_zero_blocks")
+static int g_evo_vec = -1;  // -1 = the class's, else 0 (a lane per state) or 1 (four states per lane) ("evo_vec")
 
 bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_rows_per_lane") {
@@ -371,6 +496,11 @@ bool evo_set_tuning(const std::string &key, int value) {
         g_evo_lds = value;
         return true;
     }
+    if (key == "evo_vec") {
+        if (value < -1 || value > 1) throw ArgError("evo_vec must be -1, 0 or 1");
+        g_evo_vec = value;
+        return true;
+    }
     if (key == "evo_skip_zero_blocks") {
         if (value != 0 && value != 1) throw ArgError("evo_skip_zero_blocks must be 0 or 1");
         g_evo_skip = value;
@@ -386,14 +516,36 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
     const int rpt = g_evo_rpt > 0 ? g_evo_rpt : c.rpt;
     const int xcd = g_evo_xcd >= 0 ? g_evo_xcd : c.xcd;
     const bool lds = (g_evo_lds >= 0 ? g_evo_lds : c.lds) == 1;
-    const int64_t rows_per_block = static_cast<int64_t>(RPP) * rpt;
-    const int64_t ntiles = (N + rows_per_block - 1) / rows_per_block;
-    const dim3 grid(static_cast<unsigned>(grid_for_tiles(ntiles, xcd)), static_cast<unsigned>(c.jobs_n()));
     float *bout_r = (g & 1) ? c.b1r.as<float>() : c.b0r.as<float>();
     float *bout_i = (g & 1) ? c.b1i.as<float>() : c.b0i.as<float>();
     const float *bin_r = (g & 1) ? c.b0r.as<float>() : c.b1r.as<float>();
     const float *bin_i = (g & 1) ? c.b0i.as<float>() : c.b1i.as<float>();
     const uint32_t *zmask = g_evo_skip ? c.zmask.as<uint32_t>() : nullptr;
+    if constexpr (NS == 4 || NS == 8 || NS == 16) {
+        if ((g_evo_vec >= 0 ? g_evo_vec : c.vec) == 1) {  // four states per lane (no LDS window)
+            const int64_t rows4 = static_cast<int64_t>(1024 / NS) * rpt;
+            const int64_t nt4 = (N + rows4 - 1) / rows4;
+            const dim3 grid4(static_cast<unsigned>(grid_for_tiles(nt4, xcd)), static_cast<unsigned>(c.jobs_n()));
+#define DQ_HORNER4(R)                                                                                              \
+    hipLaunchKernelGGL((horner4_kernel<TV, NS, R>), grid4, dim3(256), 0, s, N, (int)c.offs.size(),                  \
+                       c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
+                       g, c.p64r.as<double>(), c.p64i.as<double>(), c.p32r.as<float>(), c.p32i.as<float>(), bin_r,   \
+                       bin_i, bout_r, bout_i, nt4, xcd, zmask, c.zwords)
+            switch (rpt) {
+                case 1: DQ_HORNER4(1); break;
+                case 2: DQ_HORNER4(2); break;
+                case 4: DQ_HORNER4(4); break;
+                case 8: DQ_HORNER4(8); break;
+                default: throw ArgError("evolve: rows per lane must be 1, 2, 4 or 8");
+            }
+#undef DQ_HORNER4
+            DQ_LAUNCH_CHECK();
+            return;
+        }
+    }
+    const int64_t rows_per_block = static_cast<int64_t>(RPP) * rpt;
+    const int64_t ntiles = (N + rows_per_block - 1) / rows_per_block;
+    const dim3 grid(static_cast<unsigned>(grid_for_tiles(ntiles, xcd)), static_cast<unsigned>(c.jobs_n()));
 #define DQ_HORNER_L(R, L)                                                                                        \
     hipLaunchKernelGGL((horner_kernel<TV, NS, R, L>), grid, dim3(256), 0, s, N, (int)c.offs.size(),               \
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
@@ -482,8 +634,8 @@ static void run_class_levels(dq_evo &E, EvoClass &c, hipStream_t s, bool renorm)
 // class its number of levels and its tuned schedule. The kernels read each job's (tau, m, s) from d_params when they
 // run, so a new schedule that keeps every class's number of levels replays the same graph.
 static std::vector<int64_t> graph_signature(const dq_evo &E, bool renorm) {
-    std::vector<int64_t> sig{renorm ? 1 : 0, g_evo_rpt, g_evo_xcd, g_evo_lds, g_evo_skip};
-    for (const auto &c : E.classes) sig.insert(sig.end(), {c.G, c.rpt, c.xcd, c.lds});
+    std::vector<int64_t> sig{renorm ? 1 : 0, g_evo_rpt, g_evo_xcd, g_evo_lds, g_evo_skip, g_evo_vec};
+    for (const auto &c : E.classes) sig.insert(sig.end(), {c.G, c.rpt, c.xcd, c.lds, c.vec});
     return sig;
 }
 
@@ -681,11 +833,17 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *cho
             DevBuf fr(sv * 4), fi(sv * 4);
             DQ_HIP(hipMemcpy(fr.p, c.p32r.p, sv * 4, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(fi.p, c.p32i.p, sv * 4, hipMemcpyDeviceToDevice));
-            for (int cand = 0; cand < 16; ++cand) {
-                const int rpt = 1 << (cand & 3), xcd = (cand >> 2) & 1, lds = cand >> 3;
+            const bool vec_ok = E->n_states == 4 || E->n_states == 8 || E->n_states == 16;
+            int bvec = c.vec;
+            for (int cand = 0; cand < 24; ++cand) {
+                // 0-15: a lane per state (rows per lane x tile order x LDS window); 16-23: four states per lane
+                const int vec = cand >= 16 ? 1 : 0;
+                if (vec && !vec_ok) break;
+                const int rpt = 1 << (cand & 3), xcd = (cand >> 2) & 1, lds = vec ? 0 : cand >> 3;
                 c.rpt = rpt;
                 c.xcd = xcd;
                 c.lds = lds;
+                c.vec = vec;
                 dispatch(E->value_precision, E->n_states, [&](auto tv, auto ns) {
                     using TV = decltype(tv);
                     constexpr int NS = decltype(ns)::value;
@@ -697,14 +855,15 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *cho
                 DQ_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; blds = lds; }
+                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; blds = lds; bvec = vec; }
             }
             c.rpt = brpt;
             c.xcd = bxcd;
             c.lds = blds;
+            c.vec = bvec;
             if (chosen_rpt) chosen_rpt[ci] = brpt;
             if (chosen_xcd) chosen_xcd[ci] = bxcd;
-            if (chosen_lds) chosen_lds[ci] = blds;
+            if (chosen_lds) chosen_lds[ci] = bvec ? 2 : blds;
             DQ_HIP(hipMemcpy(c.p64r.p, sr.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p64i.p, si.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p32r.p, fr.p, sv * 4, hipMemcpyDeviceToDevice));

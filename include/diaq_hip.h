@@ -184,7 +184,8 @@ int dq_release_cached_memory(void);
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
  * product's values) and "product_rows_per_lane" (4, 8 = default, 16: row chunks of 256 x that) set the SpGEMM
  * kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
- * (-1 = tuned, 0, 1) and "evo_lds" (-1 = tuned, 0, 1) override every batched-evolution class's Horner schedule,
+ * (-1 = tuned, 0, 1), "evo_lds" (-1 = tuned, 0, 1) and "evo_vec" (-1 = tuned, 0 = a lane per state, 1 = four states
+ * per lane for 4, 8 or 16 states) override every batched-evolution class's Horner schedule,
  * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
  * "spmv_skip_zero_blocks" (1 = default, 0) and "hamsim_skip_zero_blocks" (1 = default, 0) do the same for the
  * SpMV (the map dq_spmv_tune keeps, and spMV_ntimes's own) and for the fused evolution's Taylor terms, whose schedule
@@ -374,8 +375,10 @@ int dq_evo_get_states(const dq_evo *E, int job, double *re, double *im);
 int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int32_t *s);
 
 /* Rows per lane, row-tile order (0 = round-robin over the XCDs, 1 = each XCD a contiguous eighth of the rows) and
- * LDS window (1 = each tile stages its input rows +-T/2 in LDS) of every class by timing the candidates (after the
- * schedule is set); chosen_rpt / chosen_xcd / chosen_lds (may be NULL) receive one value per class. */
+ * kernel form of every class by timing the candidates (after the schedule is set); chosen_rpt / chosen_xcd /
+ * chosen_lds (may be NULL) receive one value per class. chosen_lds: 0 = a lane per state, 1 = a lane per state with
+ * the LDS window (each tile stages its input rows +-T/2 in LDS), 2 = four states per lane (16-byte gathers; plans
+ * of 4, 8 or 16 states). Every form gives the same bits. */
 int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *chosen_lds);
 
 /* A-posteriori truncation estimate per job from its current (initial) states: 2^s ||t_{m+1}|| / (1 - ratio) of the

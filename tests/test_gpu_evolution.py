@@ -121,15 +121,17 @@ def test_probe_flags_too_small_degree(mods):
     assert np.all(est_bad > 1e-6), est_bad
 
 
-def test_every_horner_schedule_gives_the_same_bits(mods):
-    """Rows per lane, the row-tile order and the LDS window only change which workgroup computes a row and where its
-    inputs come from, not the row's arithmetic: every (rows per lane, XCD order, LDS window) schedule, forced through
-    the process-wide knobs, reproduces the default run bit for bit, and so does the tuned plan."""
+@pytest.mark.parametrize("n_states", [4, 8])
+def test_every_horner_schedule_gives_the_same_bits(mods, n_states):
+    """Rows per lane, the row-tile order, the LDS window and the kernel form (a lane per state, or four states per
+    lane with 16-byte gathers) only change which lane computes a (row, state) and where its inputs come from, not its
+    arithmetic: every schedule, forced through the process-wide knobs, reproduces the default run bit for bit, and so
+    does the tuned plan."""
     from diaq_amd import capi
     d, ev = mods
-    jobs = small_batch(d, seed=3, n_states=4)
+    jobs = small_batch(d, seed=3, n_states=n_states)
     sched = [ev.plain_rule(d.estimate_1norm(j["D"]), j["t"], 1e-8) for j in jobs]
-    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=4, value_precision="fp32")
+    B = ev.BatchedEvolution([j["D"] for j in jobs], n_states=n_states, value_precision="fp32")
     B.set_schedule([j["t"] for j in jobs], [m for m, s in sched], [s for m, s in sched])
     L = capi.lib()
 
@@ -139,22 +141,26 @@ def test_every_horner_schedule_gives_the_same_bits(mods):
         B.run(use_graph=False, renormalize=True)
         return [B.get_states(q, 1 << j["n"]) for q, j in enumerate(jobs)]
 
+    capi.check(L.dq_set_tuning(b"evo_vec", 0))
     base = run()
     try:
-        for rpt in (1, 2, 4, 8):
-            for xcd in (0, 1):
-                for lds in (0, 1):
-                    capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
-                    capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
-                    capi.check(L.dq_set_tuning(b"evo_lds", lds))
-                    for a, b in zip(run(), base):
-                        np.testing.assert_array_equal(a, b, err_msg=f"rpt={rpt} xcd={xcd} lds={lds}")
+        for vec in (0, 1):
+            for rpt in (1, 2, 4, 8):
+                for xcd in (0, 1):
+                    for lds in ((0, 1) if vec == 0 else (0,)):
+                        capi.check(L.dq_set_tuning(b"evo_vec", vec))
+                        capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+                        capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
+                        capi.check(L.dq_set_tuning(b"evo_lds", lds))
+                        for a, b in zip(run(), base):
+                            np.testing.assert_array_equal(a, b, err_msg=f"vec={vec} rpt={rpt} xcd={xcd} lds={lds}")
     finally:
         capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
         capi.check(L.dq_set_tuning(b"evo_xcd", -1))
         capi.check(L.dq_set_tuning(b"evo_lds", -1))
+        capi.check(L.dq_set_tuning(b"evo_vec", -1))
     chosen = B.tune()
-    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) and w in (0, 1)
+    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) and w in (0, 1, 2)
                                               for r, x, w in chosen)
     for a, b in zip(run(), base):
         np.testing.assert_array_equal(a, b)
@@ -192,13 +198,16 @@ def test_zero_block_skip_matches_full_reads(mods):
         capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 0))
         full = run()
         capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 1))
-        for rpt in (1, 8):
-            capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
-            for a, b in zip(run(), skip):
-                np.testing.assert_array_equal(a, b)
+        for vec in (0, 1):
+            for rpt in (1, 8):
+                capi.check(L.dq_set_tuning(b"evo_vec", vec))
+                capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+                for a, b in zip(run(), skip):
+                    np.testing.assert_array_equal(a, b)
     finally:
         capi.check(L.dq_set_tuning(b"evo_skip_zero_blocks", 1))
         capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
+        capi.check(L.dq_set_tuning(b"evo_vec", -1))
     for a, b in zip(skip, full):
         assert np.max(np.abs(a - b)) < 1e-6
     for j, got in zip(jobs, skip):
