@@ -351,6 +351,92 @@ __global__ __launch_bounds__(256) void probe_kernel(int64_t N, int n, const Diag
     }
 }
 
+// The probe with four states per lane (NS = 4, 8 or 16), the gathers as 16-byte loads, as horner4_kernel; R rows per
+// lane, tiles of 4 R zero-map blocks.
+template <typename TV, int NS, int R>
+__global__ __launch_bounds__(256) void probe4_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
+                                                     int64_t total, const TV *__restrict__ hre,
+                                                     const TV *__restrict__ him, const JobParam *__restrict__ params,
+                                                     int kstep, const float *__restrict__ p32r,
+                                                     const float *__restrict__ p32i, const float *__restrict__ tin_r,
+                                                     const float *__restrict__ tin_i, float *__restrict__ tout_r,
+                                                     float *__restrict__ tout_i, double *__restrict__ norms,
+                                                     const uint32_t *__restrict__ zmask, int64_t zwords) {
+    static_assert(NS == 4 || NS == 8 || NS == 16, "four states per lane");
+    constexpr int L = NS / 4, WR = 64 / L, RPP = 4 * WR, T = RPP * R;
+    const int j = blockIdx.y;
+    const JobParam P = params[j];
+    if (kstep > P.m + 1) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane % L, rw = lane / L;
+    const int64_t t = blockIdx.x;
+    const int64_t tile = t * T;
+    const int64_t vb = static_cast<int64_t>(j) * total;
+    const int64_t sb = static_cast<int64_t>(j) * N * NS;
+    const float *xr = (kstep == 1) ? p32r + sb : tin_r + sb;
+    const float *xi = (kstep == 1) ? p32i + sb : tin_i + sb;
+    const int64_t blk0 = t * 4 * R;
+    Acc<TV> hr[R][4], hi[R][4];
+    for (int d = 0; d < n; ++d) {
+        const DiagMeta md = meta[d];
+        const int64_t lo = md.rlo, hi_ = md.rhi;
+        if (hi_ <= tile || lo >= tile + T) continue;  // uniform
+        uint32_t live = (4 * R >= 32) ? ~0u : ((1u << (4 * R)) - 1);
+        if (zmask) {
+            const uint32_t w = zmask[(static_cast<int64_t>(j) * n + d) * zwords + (blk0 >> 5)];
+            live = (4 * R >= 32) ? w : ((w >> (blk0 & 31)) & ((1u << (4 * R)) - 1));
+            if (!live) continue;  // uniform
+        }
+        const TV *ar = hre + vb + md.rowbase;
+        const TV *ai = him + vb + md.rowbase;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (!((live >> (4 * q + wave)) & 1u)) continue;  // uniform in the wave
+            const int64_t r = tile + RPP * q + WR * wave + rw;
+            const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);
+            const TV a_r = ar[rc], a_i = ai[rc];
+            const int64_t xo = (rc + md.off) * NS + 4 * h;
+            const float4 x_r = *reinterpret_cast<const float4 *>(xr + xo);
+            const float4 x_i = *reinterpret_cast<const float4 *>(xi + xo);
+            if (r >= lo && r < hi_) {
+                cmac(hr[q][0], hi[q][0], a_r, a_i, static_cast<TV>(x_r.x), static_cast<TV>(x_i.x));
+                cmac(hr[q][1], hi[q][1], a_r, a_i, static_cast<TV>(x_r.y), static_cast<TV>(x_i.y));
+                cmac(hr[q][2], hi[q][2], a_r, a_i, static_cast<TV>(x_r.z), static_cast<TV>(x_i.z));
+                cmac(hr[q][3], hi[q][3], a_r, a_i, static_cast<TV>(x_r.w), static_cast<TV>(x_i.w));
+            }
+        }
+    }
+    const double c = P.tau / static_cast<double>(kstep);
+    double sq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int64_t r = tile + RPP * q + WR * wave + rw;
+        if (r >= N) continue;
+        const int64_t o = sb + r * NS + 4 * h;
+        float vr32[4], vi32[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double vr = c * static_cast<double>(hi[q][v].get()), vi = -c * static_cast<double>(hr[q][v].get());
+            vr32[v] = static_cast<float>(vr);
+            vi32[v] = static_cast<float>(vi);
+            sq[v] += vr * vr + vi * vi;
+        }
+        *reinterpret_cast<float4 *>(tout_r + o) = make_float4(vr32[0], vr32[1], vr32[2], vr32[3]);
+        *reinterpret_cast<float4 *>(tout_i + o) = make_float4(vi32[0], vi32[1], vi32[2], vi32[3]);
+    }
+    if (kstep >= P.m) {
+        // lanes holding the same four states are L apart in the wave: fold them onto lanes 0 .. L-1
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int w = 32; w >= L; w >>= 1) sq[v] += __shfl_xor(sq[v], w, 64);
+        if (lane < L)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                atomicAdd(&norms[(static_cast<int64_t>(j) * NS + 4 * h + v) * 2 + (kstep - P.m)], sq[v]);
+    }
+}
+
 // Squared norms of every state (FP64): per-workgroup partials partial[(j * gridDim.x + block) * NS + state], folded in
 // a fixed order by fold_states_kernel (no atomics: every replay gives the same bits), then psi /= ||psi||.
 template <int NS>
@@ -587,11 +673,22 @@ static void launch_probe(const EvoClass &c, int kstep, hipStream_t s) {
     const int64_t N = c.N;
     constexpr int RPP = 256 / NS;
     constexpr int R = 4;
-    const dim3 grid(static_cast<unsigned>((N + RPP * R - 1) / (RPP * R)), static_cast<unsigned>(c.jobs_n()));
     float *tout_r = (kstep & 1) ? c.b1r.as<float>() : c.b0r.as<float>();
     float *tout_i = (kstep & 1) ? c.b1i.as<float>() : c.b0i.as<float>();
     const float *tin_r = (kstep & 1) ? c.b0r.as<float>() : c.b1r.as<float>();
     const float *tin_i = (kstep & 1) ? c.b0i.as<float>() : c.b1i.as<float>();
+    if constexpr (NS == 4 || NS == 8 || NS == 16) {  // four states per lane: a quarter of the load instructions
+        constexpr int T4 = (1024 / NS) * R;
+        const dim3 grid4(static_cast<unsigned>((N + T4 - 1) / T4), static_cast<unsigned>(c.jobs_n()));
+        hipLaunchKernelGGL((probe4_kernel<TV, NS, R>), grid4, dim3(256), 0, s, N, (int)c.offs.size(),
+                           c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(),
+                           c.d_params.as<JobParam>(), kstep, c.p32r.as<float>(), c.p32i.as<float>(), tin_r, tin_i,
+                           tout_r, tout_i, c.d_probe.as<double>(), g_evo_skip ? c.zmask.as<uint32_t>() : nullptr,
+                           c.zwords);
+        DQ_LAUNCH_CHECK();
+        return;
+    }
+    const dim3 grid(static_cast<unsigned>((N + RPP * R - 1) / (RPP * R)), static_cast<unsigned>(c.jobs_n()));
     hipLaunchKernelGGL((probe_kernel<TV, NS, R>), grid, dim3(256), 0, s, N, (int)c.offs.size(),
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(),
                        kstep, c.p32r.as<float>(), c.p32i.as<float>(), tin_r, tin_i, tout_r, tout_i,
