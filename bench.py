@@ -1339,7 +1339,11 @@ def main():
                                        "a 2 GiB copy (profiles/spmv_fp64_n2e24_traffic.json), not measured in this run",
                      "launch_ms": {"mean": round(sp["ms"], 4), "median": round(sp["ms_median"], 4),
                                    "min": round(sp["ms_min"], 4), "max": round(sp["ms_max"], 4), "n": args.steps},
-                     "achieved_at_median_launch": round(sp["bytes"] / (sp["ms_median"] * 1e-3) / 1e9, 2)},
+                     "achieved_at_median_launch": round(sp["bytes"] / (sp["ms_median"] * 1e-3) / 1e9, 2),
+                     "binding": "L2-miss (fill) rate: 1.912e8 misses of 128 B per launch at 6.0e10/s with a "
+                                "~1,400-cycle loaded latency and <1 % DRAM credit stalls; x is gathered once per "
+                                "diagonal (9 % L2 hits), 1.43x the misses of the structured-offset matrix "
+                                "(profiles/spmv_fp64_n2e24_counters.txt)"},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")} if "value" in cpu
                          else None),
         "spmv_fp32": sp.get("fp32"),
