@@ -19,6 +19,13 @@
 // share its L2. A workgroup walks its diagonal's pairs in batches of 256: the lanes load the batch's pair metadata
 // in parallel and leave each pair's row range and base pointers in LDS, then the pair loop reads them from there
 // (no chain of dependent metadata loads per pair).
+//
+// Zero blocks: for large operands the product first maps which 256-row blocks of each diagonal of A and B hold a
+// nonzero value (zero_blocks_kernel), and a pair whose A rows, or whose B rows, in the item's chunk are all zero is
+// dropped when its metadata is staged. Its products are all +-0, and a sum that starts at +0 is left unchanged by
+// adding +-0, so the result has the same bits; only a NaN or Inf facing a zero block is not propagated. A qubit
+// Hamiltonian's +-2^k flip diagonals are zero on alternate runs of 2^k rows, so for k at least the chunk's log2 size
+// half of a far pair's chunks read nothing -- and far pairs' B rows are the reads no cache catches.
 #include <algorithm>
 #include <atomic>
 #include <memory>
@@ -150,6 +157,7 @@ constexpr int PROD_BATCH = 256;
 static int g_prod_interleave = 1;
 static int g_prod_nt_store = 1;
 static int g_prod_rpt = 8;  // rows per lane: row chunks of 256 * rpt rows
+static int g_prod_skip = 1; // drop pairs whose rows in a chunk are all zero, for operands of >= 2^16 rows
 
 // Item list: the i-th chunk in processing order (row chunk q_of[i]) owns items [first[i], first[i + 1]), result
 // diagonals c_lo[i], c_lo[i] + 1, ...
@@ -196,6 +204,19 @@ __device__ inline int64_t uniform64(int64_t v) {
     return static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo));
 }
 
+// Whether a 256-row block of the map row z (one diagonal; bit b of word w covers rows 256 (32 w + b) ..) that meets
+// rows [lo, hi) holds a nonzero value.
+__device__ inline bool blocks_live(const uint32_t *__restrict__ z, int64_t lo, int64_t hi) {
+    const int64_t b0 = lo >> 8, b1 = (hi - 1) >> 8;
+    for (int64_t w = b0 >> 5; w <= (b1 >> 5); ++w) {
+        uint32_t m = z[w];
+        if (w == (b0 >> 5)) m &= 0xffffffffu << (b0 & 31);
+        if (w == (b1 >> 5)) m &= 0xffffffffu >> (31 - (b1 & 31));
+        if (m) return true;
+    }
+    return false;
+}
+
 template <typename T, bool NTS, int PROD_RPT>
 __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ items, const int64_t *__restrict__ xcd_first,
                                                       const int64_t *__restrict__ pair_start,
@@ -204,7 +225,9 @@ __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ i
                                                       const DiagMeta *__restrict__ mc, const T *__restrict__ are,
                                                       const T *__restrict__ aim, const T *__restrict__ bre,
                                                       const T *__restrict__ bim, T *__restrict__ cre,
-                                                      T *__restrict__ cim, double divisor) {
+                                                      T *__restrict__ cim, double divisor,
+                                                      const uint32_t *__restrict__ za, int64_t zwa,
+                                                      const uint32_t *__restrict__ zb, int64_t zwb) {
     constexpr int PROD_CHUNK = 256 * PROD_RPT;
     __shared__ int64_t s_abase[PROD_BATCH], s_bbase[PROD_BATCH];
     __shared__ int32_t s_lo[PROD_BATCH], s_hi[PROD_BATCH];
@@ -224,15 +247,19 @@ __global__ __launch_bounds__(256) void product_kernel(const int2 *__restrict__ i
         const int nb = static_cast<int>(min<int64_t>(PROD_BATCH, p1 - pbat));
         if (static_cast<int>(threadIdx.x) < nb) {
             const int64_t p = pbat + threadIdx.x;
-            const DiagMeta ta = ma[pa[p]];
-            const DiagMeta tb = mb[pb[p]];
+            const int32_t sa = pa[p], sb = pb[p];
+            const DiagMeta ta = ma[sa];
+            const DiagMeta tb = mb[sb];
             // rows r of this item with r on A's diagonal and r + dA on B's: [lo, hi), as offsets from q0
             const int64_t lo = max(max(ta.rlo, tb.rlo - ta.off), r0);
             const int64_t hi = min(min(ta.rhi, tb.rhi - ta.off), r1);
+            bool live = hi > lo;
+            if (live && za) live = blocks_live(za + static_cast<int64_t>(sa) * zwa, lo, hi);
+            if (live && zb) live = blocks_live(zb + static_cast<int64_t>(sb) * zwb, lo + ta.off, hi + ta.off);
             s_abase[threadIdx.x] = ta.rowbase;
             s_bbase[threadIdx.x] = tb.rowbase + ta.off;
-            s_lo[threadIdx.x] = hi > lo ? static_cast<int32_t>(lo - q0) : 0;
-            s_hi[threadIdx.x] = hi > lo ? static_cast<int32_t>(hi - q0) : 0;
+            s_lo[threadIdx.x] = live ? static_cast<int32_t>(lo - q0) : 0;
+            s_hi[threadIdx.x] = live ? static_cast<int32_t>(hi - q0) : 0;
         }
         __syncthreads();
         for (int k = 0; k < nb; ++k) {
@@ -363,6 +390,29 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     DQ_HIP(hipMemcpyAsync(dps.p, d.pair_start.data(), dps.bytes, hipMemcpyHostToDevice, s));
     DQ_HIP(hipMemcpyAsync(dsa.p, d.sa.data(), sizeof(int32_t) * d.sa.size(), hipMemcpyHostToDevice, s));
     DQ_HIP(hipMemcpyAsync(dsb.p, d.sb.data(), sizeof(int32_t) * d.sb.size(), hipMemcpyHostToDevice, s));
+    // zero-block maps of A and B (a squaring maps its operand once; a map the handle keeps is used as it is)
+    DevBuf zab, zbb;
+    const uint32_t *za = nullptr, *zb = nullptr;
+    const int64_t zwa = zero_block_words(A.rows, 256), zwb = zero_block_words(B.rows, 256);
+    if (g_prod_skip && A.rows >= (int64_t(1) << 16) && A.n() > 0 && B.n() > 0) {
+        auto map_of = [&](const dq_devmat &M, int64_t zw, DevBuf &buf) -> const uint32_t * {
+            if (M.zmap_valid && M.zmap.p && M.zmap_words == zw) return M.zmap.as<uint32_t>();
+            buf = DevBuf(sizeof(uint32_t) * static_cast<size_t>(zw) * static_cast<size_t>(M.n()));
+            const dim3 g(static_cast<unsigned>(zw), zero_map_grid_y(M.n()), 1);
+            if (M.precision == DQ_F64)
+                hipLaunchKernelGGL((zero_blocks_kernel<double, 256>), g, dim3(256), 0, s, static_cast<int64_t>(M.rows),
+                                   M.n(), M.d_meta.as<DiagMeta>(), int64_t(0), M.re.as<double>(), M.im.as<double>(),
+                                   zw, buf.as<uint32_t>());
+            else
+                hipLaunchKernelGGL((zero_blocks_kernel<float, 256>), g, dim3(256), 0, s, static_cast<int64_t>(M.rows),
+                                   M.n(), M.d_meta.as<DiagMeta>(), int64_t(0), M.re.as<float>(), M.im.as<float>(), zw,
+                                   buf.as<uint32_t>());
+            DQ_LAUNCH_CHECK();
+            return buf.as<uint32_t>();
+        };
+        za = map_of(A, zwa, zab);
+        zb = (&A == &B) ? za : map_of(B, zwb, zbb);
+    }
     hipLaunchKernelGGL(product_items_kernel, dim3(static_cast<unsigned>(std::min<int64_t>(n_chunks, 65536))),
                        dim3(256), 0, s, n_chunks, dfirst.as<int64_t>(), dclo.as<int32_t>(), dqof.as<int32_t>(),
                        ditems.as<int2>());
@@ -371,7 +421,8 @@ std::unique_ptr<dq_devmat> product(const dq_devmat &A, const dq_devmat &B, doubl
     hipLaunchKernelGGL((product_kernel<T, NTS, R>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,           \
                        ditems.as<int2>(), dxcd.as<int64_t>(), dps.as<int64_t>(), dsa.as<int32_t>(),               \
                        dsb.as<int32_t>(), A.d_meta.as<DiagMeta>(), B.d_meta.as<DiagMeta>(), C->d_meta.as<DiagMeta>(), \
-                       A.re.as<T>(), A.im.as<T>(), B.re.as<T>(), B.im.as<T>(), C->re.as<T>(), C->im.as<T>(), divisor)
+                       A.re.as<T>(), A.im.as<T>(), B.re.as<T>(), B.im.as<T>(), C->re.as<T>(), C->im.as<T>(), divisor, \
+                       za, zwa, zb, zwb)
 #define DQ_PROD_LAUNCH(T, NTS)                                \
     do {                                                      \
         if (prod_rpt == 4) { DQ_PROD_LAUNCH_R(T, NTS, 4); }   \
@@ -455,6 +506,11 @@ bool product_set_tuning(const std::string &key, int value) {
     if (key == "product_rows_per_lane") {
         if (value != 4 && value != 8 && value != 16) throw ArgError("product_rows_per_lane must be 4, 8 or 16");
         g_prod_rpt = value;
+        return true;
+    }
+    if (key == "product_skip_zero_blocks") {
+        if (value < 0 || value > 1) throw ArgError("product_skip_zero_blocks must be 0 or 1");
+        g_prod_skip = value;
         return true;
     }
     if (key == "product_nt_store") {

@@ -403,6 +403,64 @@ def test_tfim20_hh_fp64_fp32(d):
     assert 0 < err < 1e-6
 
 
+@pytest.mark.parametrize("model", ["tfim", "xxz"])
+def test_product_zero_block_skip_is_exact(d, model):
+    """For operands of 2^16 rows and more the product maps their all-zero 256-row blocks and drops the pairs whose rows
+    in a chunk are all zero ("product_skip_zero_blocks"). A dropped pair's products are exact zeros, so the values
+    keep their bits with the skip on or off: for H.H (one map) and H.G (two maps, G with other values or other
+    offsets), in FP64 and FP32, for every chunk size, and with the map dq_spmv_tune keeps in an FP64 handle. The FP64
+    values agree with the oracle's product (src/spGEMM_gpu.cu:25-151) within 1e-12."""
+    from diaq_amd import capi
+    L = capi.lib()
+    q = 16
+    if model == "tfim":
+        H = Hm.tfim(q, 1.0, 0.7)
+        G = Hm.tfim(q, 0.9, 0.4)
+    else:
+        H = Hm.xxz(q, seed=6)
+        G = Hm.tfim(q, 1.1, 0.5)
+    ref = {}
+    for name, (X, Y) in (("HH", (H, H)), ("HG", (H, G))):
+        R = O.product(O.Dia(*X), O.Dia(*Y))
+        ref[name] = dict(zip(R.offsets, R.vals))
+
+    def values(C):
+        return {k: C.get_diagonal(k) for k in C.diagonals}
+
+    def products(prec):
+        DH = d.to_device(dia_of(d, *H), precision=prec)
+        DG = d.to_device(dia_of(d, *G), precision=prec)
+        return DH, {"HH": values(d.product(DH, DH)), "HG": values(d.product(DH, DG))}
+
+    try:
+        for prec in ("fp64", "fp32"):
+            capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 0))
+            capi.check(L.dq_set_tuning(b"product_rows_per_lane", 8))
+            _, base = products(prec)
+            capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 1))
+            for rpt in (4, 8, 16):
+                capi.check(L.dq_set_tuning(b"product_rows_per_lane", rpt))
+                DH, got = products(prec)
+                for name in base:
+                    assert list(got[name]) == list(base[name]), (prec, rpt, name)
+                    for k in base[name]:
+                        np.testing.assert_array_equal(got[name][k], base[name][k], err_msg=f"{prec} {rpt} {name} {k}")
+            if prec == "fp64":
+                for name in base:
+                    for k, v in base[name].items():
+                        assert rel(v, ref[name][k]) < TOL, (name, k)
+                capi.check(L.dq_set_tuning(b"product_rows_per_lane", 8))
+                capi.check(L.dq_spmv_tune(DH.handle, 2, 0, None))  # the handle now keeps its zero-block map
+                tuned = values(d.product(DH, DH))
+                for k in base["HH"]:
+                    np.testing.assert_array_equal(tuned[k], base["HH"][k], err_msg=f"tuned map {k}")
+    finally:
+        capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 1))
+        capi.check(L.dq_set_tuning(b"product_rows_per_lane", 8))
+    with pytest.raises(RuntimeError, match="product_skip_zero_blocks"):
+        capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 2))
+
+
 # ---------------------------------------------------------------------------------------------------------------------
 # SpME
 @pytest.mark.parametrize("n,t", [(6, 0.1), (8, 0.7), (9, 2.5)])
