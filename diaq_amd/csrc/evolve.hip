@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <numeric>
 #include <vector>
 
 #include "dia_rows.hpp"
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(256) void horner_kernel(int64_t N, int n, const Dia
 // wave covers 256 / NS rows per pass -- exactly one block of the zero-block map -- and the workgroup RPT passes of
 // 1024 / NS rows. Each (row, state) sums its diagonals in the same order with the same operations as horner_kernel,
 // so the two kernels give the same bits.
-template <typename TV, int NS, int RPT>
+template <typename TV, int NS, int RPT, int DU = 1>
 __global__ __launch_bounds__(256) void horner4_kernel(int64_t N, int n, const DiagMeta *__restrict__ meta,
                                                       int64_t total, const TV *__restrict__ hre,
                                                       const TV *__restrict__ him, const JobParam *__restrict__ params,
@@ -200,41 +201,68 @@ __global__ __launch_bounds__(256) void horner4_kernel(int64_t N, int n, const Di
     const float *xi = bin_i + sb;
     const int64_t blk0 = t * 4 * RPT;  // the tile's first map block
     Acc<TV> hr[RPT][4], hi[RPT][4];
-    for (int d = 0; d < n; ++d) {
-        const DiagMeta md = meta[d];
-        const int64_t lo = md.rlo, hi_ = md.rhi;
-        if (hi_ <= tile || lo >= tile + T) continue;  // uniform
-        uint32_t live = (4 * RPT >= 32) ? ~0u : ((1u << (4 * RPT)) - 1);
-        if (zmask) {
-            const uint32_t w = zmask[(static_cast<int64_t>(j) * n + d) * zwords + (blk0 >> 5)];
-            live = (4 * RPT >= 32) ? w : ((w >> (blk0 & 31)) & ((1u << (4 * RPT)) - 1));
-            if (!live) continue;  // uniform: every value of the diagonal in this tile is zero
-        }
-        const TV *ar = hre + vb + md.rowbase;
-        const TV *ai = him + vb + md.rowbase;
-        TV va_r[RPT] = {}, va_i[RPT] = {};
-        float4 vx_r[RPT], vx_i[RPT];
+    // DU diagonals per batch: their loads are all issued before the first of them is used (more loads in flight per
+    // wave); the products are still added diagonal after diagonal, in slot order
+    for (int d0 = 0; d0 < n; d0 += DU) {
+        TV va_r[DU][RPT], va_i[DU][RPT];
+        float4 vx_r[DU][RPT], vx_i[DU][RPT];
+        uint32_t lv[DU];
+        int64_t lo_[DU], hi_[DU];
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            vx_r[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-            vx_i[q] = vx_r[q];
-            if (!((live >> (4 * q + wave)) & 1u)) continue;  // uniform in the wave
-            const int64_t r = tile + RPP * q + WR * wave + rw;
-            const int64_t rc = r < lo ? lo : (r >= hi_ ? hi_ - 1 : r);  // stays inside the tile
-            va_r[q] = ar[rc];
-            va_i[q] = ai[rc];
-            const int64_t xo = (rc + md.off) * NS + 4 * h;
-            vx_r[q] = *reinterpret_cast<const float4 *>(xr + xo);
-            vx_i[q] = *reinterpret_cast<const float4 *>(xi + xo);
+        for (int u = 0; u < DU; ++u) {
+            lv[u] = 0;
+            lo_[u] = 0;
+            hi_[u] = 0;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                va_r[u][q] = TV(0);
+                va_i[u][q] = TV(0);
+                vx_r[u][q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                vx_i[u][q] = vx_r[u][q];
+            }
+            const int d = d0 + u;
+            if (d >= n) continue;  // uniform
+            const DiagMeta md = meta[d];
+            const int64_t rlo = md.rlo, rhi = md.rhi;
+            if (rhi <= tile || rlo >= tile + T) continue;  // uniform
+            uint32_t live = (4 * RPT >= 32) ? ~0u : ((1u << (4 * RPT)) - 1);
+            if (zmask) {
+                const uint32_t w = zmask[(static_cast<int64_t>(j) * n + d) * zwords + (blk0 >> 5)];
+                live = (4 * RPT >= 32) ? w : ((w >> (blk0 & 31)) & ((1u << (4 * RPT)) - 1));
+                if (!live) continue;  // uniform: every value of the diagonal in this tile is zero
+            }
+            lv[u] = live;
+            lo_[u] = rlo;
+            hi_[u] = rhi;
+            const TV *ar = hre + vb + md.rowbase;
+            const TV *ai = him + vb + md.rowbase;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (!((live >> (4 * q + wave)) & 1u)) continue;  // uniform in the wave
+                const int64_t r = tile + RPP * q + WR * wave + rw;
+                const int64_t rc = r < rlo ? rlo : (r >= rhi ? rhi - 1 : r);  // stays inside the tile
+                va_r[u][q] = ar[rc];
+                va_i[u][q] = ai[rc];
+                const int64_t xo = (rc + md.off) * NS + 4 * h;
+                vx_r[u][q] = *reinterpret_cast<const float4 *>(xr + xo);
+                vx_i[u][q] = *reinterpret_cast<const float4 *>(xi + xo);
+            }
         }
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
-            const int64_t r = tile + RPP * q + WR * wave + rw;
-            if (((live >> (4 * q + wave)) & 1u) && r >= lo && r < hi_) {
-                cmac(hr[q][0], hi[q][0], va_r[q], va_i[q], static_cast<TV>(vx_r[q].x), static_cast<TV>(vx_i[q].x));
-                cmac(hr[q][1], hi[q][1], va_r[q], va_i[q], static_cast<TV>(vx_r[q].y), static_cast<TV>(vx_i[q].y));
-                cmac(hr[q][2], hi[q][2], va_r[q], va_i[q], static_cast<TV>(vx_r[q].z), static_cast<TV>(vx_i[q].z));
-                cmac(hr[q][3], hi[q][3], va_r[q], va_i[q], static_cast<TV>(vx_r[q].w), static_cast<TV>(vx_i[q].w));
+        for (int u = 0; u < DU; ++u) {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
+                const int64_t r = tile + RPP * q + WR * wave + rw;
+                if (((lv[u] >> (4 * q + wave)) & 1u) && r >= lo_[u] && r < hi_[u]) {
+                    cmac(hr[q][0], hi[q][0], va_r[u][q], va_i[u][q], static_cast<TV>(vx_r[u][q].x),
+                         static_cast<TV>(vx_i[u][q].x));
+                    cmac(hr[q][1], hi[q][1], va_r[u][q], va_i[u][q], static_cast<TV>(vx_r[u][q].y),
+                         static_cast<TV>(vx_i[u][q].y));
+                    cmac(hr[q][2], hi[q][2], va_r[u][q], va_i[u][q], static_cast<TV>(vx_r[u][q].z),
+                         static_cast<TV>(vx_i[u][q].z));
+                    cmac(hr[q][3], hi[q][3], va_r[u][q], va_i[u][q], static_cast<TV>(vx_r[u][q].w),
+                         static_cast<TV>(vx_i[u][q].w));
+                }
             }
         }
     }
@@ -532,6 +560,7 @@ struct EvoClass {
     int xcd = 0;              // tuned row-tile order: 0 round-robin over the XCDs, 1 XCD-contiguous
     int lds = 0;              // tuned: 1 = stage each tile's input window in LDS
     int vec = 0;              // tuned: 1 = horner4_kernel (four states per lane; NS = 4, 8, 16, no LDS window)
+    int du = 1;               // tuned: horner4_kernel's diagonals per load batch (1, 2 or 4; 1 or 2 rows per lane)
     hipStream_t stream = nullptr;
     int jobs_n() const { return static_cast<int>(jobs.size()); }
 };
@@ -547,11 +576,13 @@ struct dq_evo {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<int64_t> graph_sig;  // what the built graph's launches depend on (graph_signature)
+    std::vector<hipStream_t> lanes;  // streams of the lane plan (lane_plan), made on first use
     ~dq_evo() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &c : classes)
             if (c.stream) (void)hipStreamDestroy(c.stream);
+        for (auto l : lanes) (void)hipStreamDestroy(l);
     }
 };
 
@@ -564,6 +595,9 @@ static int g_evo_xcd = -1;  // -1 = the class's, else 0 or 1
 static int g_evo_lds = -1;  // -1 = the class's, else 0 or 1
 static int g_evo_skip = 1;  // skip the all-zero blocks of a diagonal ("evo_skip_zero_blocks")
 static int g_evo_vec = -1;  // -1 = the class's, else 0 (a lane per state) or 1 (four states per lane) ("evo_vec")
+static int g_evo_lanes = 4; // streams the classes are spread over in dq_evo_run; 0 = one per class ("evo_lanes")
+static int g_evo_du = 0;    // 0 = the class's, else the diagonals whose loads horner4_kernel issues together, for
+                            // 1 or 2 rows per lane ("evo_diag_batch": 0, 1, 2 or 4)
 
 bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_rows_per_lane") {
@@ -585,6 +619,17 @@ bool evo_set_tuning(const std::string &key, int value) {
     if (key == "evo_vec") {
         if (value < -1 || value > 1) throw ArgError("evo_vec must be -1, 0 or 1");
         g_evo_vec = value;
+        return true;
+    }
+    if (key == "evo_diag_batch") {
+        if (value != 0 && value != 1 && value != 2 && value != 4)
+            throw ArgError("evo_diag_batch must be 0, 1, 2 or 4");
+        g_evo_du = value;
+        return true;
+    }
+    if (key == "evo_lanes") {
+        if (value < 0 || value > 8) throw ArgError("evo_lanes must be 0 (a stream per class) or 1 to 8");
+        g_evo_lanes = value;
         return true;
     }
     if (key == "evo_skip_zero_blocks") {
@@ -612,16 +657,21 @@ static void launch_level(const EvoClass &c, int64_t g, hipStream_t s) {
             const int64_t rows4 = static_cast<int64_t>(1024 / NS) * rpt;
             const int64_t nt4 = (N + rows4 - 1) / rows4;
             const dim3 grid4(static_cast<unsigned>(grid_for_tiles(nt4, xcd)), static_cast<unsigned>(c.jobs_n()));
-#define DQ_HORNER4(R)                                                                                              \
-    hipLaunchKernelGGL((horner4_kernel<TV, NS, R>), grid4, dim3(256), 0, s, N, (int)c.offs.size(),                  \
+#define DQ_HORNER4(R, DU)                                                                                          \
+    hipLaunchKernelGGL((horner4_kernel<TV, NS, R, DU>), grid4, dim3(256), 0, s, N, (int)c.offs.size(),              \
                        c.d_meta.as<DiagMeta>(), c.total, c.hre.as<TV>(), c.him.as<TV>(), c.d_params.as<JobParam>(), \
                        g, c.p64r.as<double>(), c.p64i.as<double>(), c.p32r.as<float>(), c.p32i.as<float>(), bin_r,   \
                        bin_i, bout_r, bout_i, nt4, xcd, zmask, c.zwords)
-            switch (rpt) {
-                case 1: DQ_HORNER4(1); break;
-                case 2: DQ_HORNER4(2); break;
-                case 4: DQ_HORNER4(4); break;
-                case 8: DQ_HORNER4(8); break;
+            const int du = rpt <= 2 ? (g_evo_du > 0 ? g_evo_du : c.du) : 1;
+            switch (rpt * 8 + du) {
+                case 9: DQ_HORNER4(1, 1); break;
+                case 10: DQ_HORNER4(1, 2); break;
+                case 12: DQ_HORNER4(1, 4); break;
+                case 17: DQ_HORNER4(2, 1); break;
+                case 18: DQ_HORNER4(2, 2); break;
+                case 20: DQ_HORNER4(2, 4); break;
+                case 33: DQ_HORNER4(4, 1); break;
+                case 65: DQ_HORNER4(8, 1); break;
                 default: throw ArgError("evolve: rows per lane must be 1, 2, 4 or 8");
             }
 #undef DQ_HORNER4
@@ -727,12 +777,58 @@ static void run_class_levels(dq_evo &E, EvoClass &c, hipStream_t s, bool renorm)
     });
 }
 
-// What a captured graph's launches depend on: the renormalisation flag, the process-wide Horner overrides, and per
-// class its number of levels and its tuned schedule. The kernels read each job's (tau, m, s) from d_params when they
-// run, so a new schedule that keeps every class's number of levels replays the same graph.
+// Which classes run on which stream, in order. Branches of a graph share the process's few hardware queues, and a
+// queue runs what it is given in order, so with a stream per class the classes mostly ran one after another, the
+// longest often late (256-TFIM batch: one kernel in flight 79 % of the graph's time). Instead the classes go to
+// g_evo_lanes streams by longest-first list scheduling on an estimate of their time -- per level a launch cost, per
+// job-level the bytes of the values and the 8-byte complex states at ~3 TB/s -- and each lane runs its classes
+// longest first. g_evo_lanes = 0 keeps a stream per class.
+static std::vector<std::vector<int>> lane_plan(const dq_evo &E) {
+    const int nc = static_cast<int>(E.classes.size());
+    std::vector<std::vector<int>> plan;
+    if (g_evo_lanes == 0) {
+        for (int ci = 0; ci < nc; ++ci) plan.push_back({ci});
+        return plan;
+    }
+    std::vector<double> cost(nc, 0.0);
+    for (int ci = 0; ci < nc; ++ci) {
+        const EvoClass &c = E.classes[ci];
+        int64_t job_levels = 0;
+        for (const auto &p : c.params) job_levels += p.steps;
+        // per row: the values in live blocks (about half of the 2 x elem_bytes per diagonal stored) and per state
+        // the gathered input, psi's copy and the output
+        const double bytes_per_job_level =
+            static_cast<double>(c.N) * (static_cast<double>(elem_bytes(E.value_precision)) *
+                                            static_cast<double>(c.offs.size()) +
+                                        24.0 * E.n_states);
+        cost[ci] = 0.012 * static_cast<double>(c.G) + static_cast<double>(job_levels) * bytes_per_job_level / 3e9;
+    }
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    const int k = std::min(g_evo_lanes, std::max(1, nc));
+    plan.assign(k, {});
+    std::vector<double> load(k, 0.0);
+    for (int ci : order) {
+        const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        plan[l].push_back(ci);
+        load[l] += cost[ci];
+    }
+    return plan;
+}
+
+// What a captured graph's launches depend on: the renormalisation flag, the process-wide Horner overrides, per class
+// its number of levels and its tuned schedule, and the lane plan. The kernels read each job's (tau, m, s) from
+// d_params when they run, so a new schedule that keeps every class's number of levels and the plan replays the same
+// graph.
 static std::vector<int64_t> graph_signature(const dq_evo &E, bool renorm) {
-    std::vector<int64_t> sig{renorm ? 1 : 0, g_evo_rpt, g_evo_xcd, g_evo_lds, g_evo_skip, g_evo_vec};
-    for (const auto &c : E.classes) sig.insert(sig.end(), {c.G, c.rpt, c.xcd, c.lds, c.vec});
+    std::vector<int64_t> sig{renorm ? 1 : 0, g_evo_rpt, g_evo_xcd, g_evo_lds, g_evo_skip, g_evo_vec, g_evo_lanes,
+                             g_evo_du};
+    for (const auto &c : E.classes) sig.insert(sig.end(), {c.G, c.rpt, c.xcd, c.lds, c.vec, c.du});
+    for (const auto &lane : lane_plan(E)) {
+        sig.push_back(-1);
+        sig.insert(sig.end(), lane.begin(), lane.end());
+    }
     return sig;
 }
 
@@ -931,16 +1027,21 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *cho
             DQ_HIP(hipMemcpy(fr.p, c.p32r.p, sv * 4, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(fi.p, c.p32i.p, sv * 4, hipMemcpyDeviceToDevice));
             const bool vec_ok = E->n_states == 4 || E->n_states == 8 || E->n_states == 16;
-            int bvec = c.vec;
-            for (int cand = 0; cand < 24; ++cand) {
-                // 0-15: a lane per state (rows per lane x tile order x LDS window); 16-23: four states per lane
+            int bvec = c.vec, bdu = c.du;
+            for (int cand = 0; cand < 28; ++cand) {
+                // 0-15: a lane per state (rows per lane x tile order x LDS window); 16-23: four states per lane;
+                // 24-27: four states per lane, 1 or 2 rows per lane, the loads of 2 diagonals batched (batches of
+                // 4 time about as well on a level with every job active but ran the 256-TFIM graph ~3 ms slower)
                 const int vec = cand >= 16 ? 1 : 0;
                 if (vec && !vec_ok) break;
-                const int rpt = 1 << (cand & 3), xcd = (cand >> 2) & 1, lds = vec ? 0 : cand >> 3;
+                const int rpt = cand >= 24 ? 1 << (cand & 1) : 1 << (cand & 3);
+                const int xcd = (cand >> (cand >= 24 ? 1 : 2)) & 1, lds = vec ? 0 : cand >> 3;
+                const int du = cand >= 24 ? 2 : 1;
                 c.rpt = rpt;
                 c.xcd = xcd;
                 c.lds = lds;
                 c.vec = vec;
+                c.du = du;
                 dispatch(E->value_precision, E->n_states, [&](auto tv, auto ns) {
                     using TV = decltype(tv);
                     constexpr int NS = decltype(ns)::value;
@@ -952,15 +1053,16 @@ extern "C" int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *cho
                 DQ_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 DQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; blds = lds; bvec = vec; }
+                if (ms < best) { best = ms; brpt = rpt; bxcd = xcd; blds = lds; bvec = vec; bdu = du; }
             }
             c.rpt = brpt;
             c.xcd = bxcd;
             c.lds = blds;
             c.vec = bvec;
+            c.du = bdu;
             if (chosen_rpt) chosen_rpt[ci] = brpt;
             if (chosen_xcd) chosen_xcd[ci] = bxcd;
-            if (chosen_lds) chosen_lds[ci] = bvec ? 2 : blds;
+            if (chosen_lds) chosen_lds[ci] = bvec ? (bdu == 1 ? 2 : (bdu == 2 ? 3 : 4)) : blds;
             DQ_HIP(hipMemcpy(c.p64r.p, sr.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p64i.p, si.p, sv * 8, hipMemcpyDeviceToDevice));
             DQ_HIP(hipMemcpy(c.p32r.p, fr.p, sv * 4, hipMemcpyDeviceToDevice));
@@ -984,16 +1086,24 @@ extern "C" int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t s
         DQ_HIP(hipEventCreate(&e0));
         DQ_HIP(hipEventCreate(&e1));
         DQ_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        std::vector<hipEvent_t> joins(E->classes.size());
+        const std::vector<std::vector<int>> plan = lane_plan(*E);
+        if (g_evo_lanes > 0)
+            while (E->lanes.size() < plan.size()) {
+                hipStream_t l;
+                DQ_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+                E->lanes.push_back(l);
+            }
+        std::vector<hipEvent_t> joins(plan.size());
         for (auto &j : joins) DQ_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
         auto body = [&](hipStream_t base) {
             DQ_HIP(hipEventRecord(fork, base));
-            for (size_t ci = 0; ci < E->classes.size(); ++ci) {
-                EvoClass &c = E->classes[ci];
-                DQ_HIP(hipStreamWaitEvent(c.stream, fork, 0));
-                run_class_levels(*E, c, c.stream, renormalize != 0);
-                DQ_HIP(hipEventRecord(joins[ci], c.stream));
-                DQ_HIP(hipStreamWaitEvent(base, joins[ci], 0));
+            for (size_t li = 0; li < plan.size(); ++li) {
+                if (plan[li].empty()) continue;
+                hipStream_t ls = g_evo_lanes > 0 ? E->lanes[li] : E->classes[plan[li][0]].stream;
+                DQ_HIP(hipStreamWaitEvent(ls, fork, 0));
+                for (int ci : plan[li]) run_class_levels(*E, E->classes[ci], ls, renormalize != 0);
+                DQ_HIP(hipEventRecord(joins[li], ls));
+                DQ_HIP(hipStreamWaitEvent(base, joins[li], 0));
             }
         };
         if (use_graph) {

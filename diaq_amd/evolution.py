@@ -89,7 +89,9 @@ class BatchedEvolution:
                                                   capi.ptr(s, ctypes.c_int32)))
 
     def tune(self):
-        """Per class: (rows per lane, XCD-contiguous tile order, LDS window) chosen by timing."""
+        """Per class: (rows per lane, XCD-contiguous tile order, kernel form) chosen by timing. Form: 0 a lane per
+        state, 1 the same with the LDS window, 2 four states per lane, 3 and 4 four states per lane with the loads of
+        2 or 4 diagonals issued together."""
         rpt = (ctypes.c_int * self.n_classes)()
         xcd = (ctypes.c_int * self.n_classes)()
         lds = (ctypes.c_int * self.n_classes)()

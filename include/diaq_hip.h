@@ -186,7 +186,10 @@ int dq_release_cached_memory(void);
  * "product_skip_zero_blocks" (1 = default: for operands of at least 2^16 rows, map their all-zero 256-row blocks and
  * drop the pairs whose rows in a chunk are all zero, same bits; 0 = read every pair) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1), "evo_lds" (-1 = tuned, 0, 1) and "evo_vec" (-1 = tuned, 0 = a lane per state, 1 = four states
- * per lane for 4, 8 or 16 states) override every batched-evolution class's Horner schedule,
+ * per lane for 4, 8 or 16 states) and "evo_diag_batch" (0 = tuned, or 1, 2, 4: the diagonals whose loads the
+ * four-states-per-lane kernel issues together at 1 or 2 rows per lane) override every batched-evolution class's
+ * Horner schedule; "evo_lanes" (4 = default, 1 to 8, or 0 = a stream per class) sets over how many streams dq_evo_run
+ * spreads the classes (longest first, by an estimate of each class's time),
  * and "evo_skip_zero_blocks" (1 = default: skip the blocks of a diagonal whose values are all zero; 0 = read them);
  * "spmv_skip_zero_blocks" (1 = default, 0) and "hamsim_skip_zero_blocks" (1 = default, 0) do the same for the
  * SpMV (the map dq_spmv_tune keeps, and spMV_ntimes's own) and for the fused evolution's Taylor terms, whose schedule
@@ -379,7 +382,8 @@ int dq_evo_set_schedule(dq_evo *E, const double *t, const int32_t *m, const int3
  * kernel form of every class by timing the candidates (after the schedule is set); chosen_rpt / chosen_xcd /
  * chosen_lds (may be NULL) receive one value per class. chosen_lds: 0 = a lane per state, 1 = a lane per state with
  * the LDS window (each tile stages its input rows +-T/2 in LDS), 2 = four states per lane (16-byte gathers; plans
- * of 4, 8 or 16 states). Every form gives the same bits. */
+ * of 4, 8 or 16 states), 3 and 4 = four states per lane with the loads of 2 or 4 diagonals issued together. Every
+ * form gives the same bits. */
 int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *chosen_lds);
 
 /* A-posteriori truncation estimate per job from its current (initial) states: 2^s ||t_{m+1}|| / (1 - ratio) of the

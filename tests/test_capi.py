@@ -98,6 +98,8 @@ def test_argument_errors_need_no_gpu():
     assert rc == capi.DQ_ERR_ARG
     assert L.dq_set_tuning(b"spmv_rows_per_lane", 3) == capi.DQ_ERR_ARG
     assert L.dq_set_tuning(b"no_such_knob", 1) == capi.DQ_ERR_ARG
+    for key, bad in ((b"evo_lanes", 9), (b"evo_diag_batch", 3), (b"product_skip_zero_blocks", 2)):
+        assert L.dq_set_tuning(key, bad) == capi.DQ_ERR_ARG and key in L.dq_last_error()
     assert L.dq_spmv_device(None, None, None, None, None, 0) == capi.DQ_ERR_ARG
     assert L.dq_spmm_dia_device(0, 0, -1, 4, 4, 0, 1, 0, 0, 0) == capi.DQ_ERR_ARG
 

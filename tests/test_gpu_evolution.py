@@ -123,10 +123,11 @@ def test_probe_flags_too_small_degree(mods):
 
 @pytest.mark.parametrize("n_states", [4, 8])
 def test_every_horner_schedule_gives_the_same_bits(mods, n_states):
-    """Rows per lane, the row-tile order, the LDS window and the kernel form (a lane per state, or four states per
-    lane with 16-byte gathers) only change which lane computes a (row, state) and where its inputs come from, not its
-    arithmetic: every schedule, forced through the process-wide knobs, reproduces the default run bit for bit, and so
-    does the tuned plan."""
+    """Rows per lane, the row-tile order, the LDS window, the kernel form (a lane per state, or four states per lane
+    with 16-byte gathers, with 1, 2 or 4 diagonals per load batch) and the streams the classes run on only change which
+    lane computes a (row, state), where its inputs come from and when, not its arithmetic: every schedule, forced
+    through the process-wide knobs, reproduces the default run bit for bit, directly and through the captured graph,
+    and so does the tuned plan."""
     from diaq_amd import capi
     d, ev = mods
     jobs = small_batch(d, seed=3, n_states=n_states)
@@ -148,19 +149,34 @@ def test_every_horner_schedule_gives_the_same_bits(mods, n_states):
             for rpt in (1, 2, 4, 8):
                 for xcd in (0, 1):
                     for lds in ((0, 1) if vec == 0 else (0,)):
-                        capi.check(L.dq_set_tuning(b"evo_vec", vec))
-                        capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
-                        capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
-                        capi.check(L.dq_set_tuning(b"evo_lds", lds))
-                        for a, b in zip(run(), base):
-                            np.testing.assert_array_equal(a, b, err_msg=f"vec={vec} rpt={rpt} xcd={xcd} lds={lds}")
+                        for du in ((1, 2, 4) if vec == 1 and rpt <= 2 else (0,)):
+                            capi.check(L.dq_set_tuning(b"evo_vec", vec))
+                            capi.check(L.dq_set_tuning(b"evo_rows_per_lane", rpt))
+                            capi.check(L.dq_set_tuning(b"evo_xcd", xcd))
+                            capi.check(L.dq_set_tuning(b"evo_lds", lds))
+                            capi.check(L.dq_set_tuning(b"evo_diag_batch", du))
+                            for a, b in zip(run(), base):
+                                np.testing.assert_array_equal(
+                                    a, b, err_msg=f"vec={vec} rpt={rpt} xcd={xcd} lds={lds} du={du}")
+        capi.check(L.dq_set_tuning(b"evo_vec", 1))
+        capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 1))
+        capi.check(L.dq_set_tuning(b"evo_diag_batch", 2))
+        for lanes in (0, 1, 2, 5):
+            capi.check(L.dq_set_tuning(b"evo_lanes", lanes))
+            for q, j in enumerate(jobs):
+                B.set_states(q, j["states"])
+            B.run(use_graph=True, renormalize=True)
+            for q, j in enumerate(jobs):
+                np.testing.assert_array_equal(B.get_states(q, 1 << j["n"]), base[q], err_msg=f"lanes={lanes}")
     finally:
         capi.check(L.dq_set_tuning(b"evo_rows_per_lane", 0))
         capi.check(L.dq_set_tuning(b"evo_xcd", -1))
         capi.check(L.dq_set_tuning(b"evo_lds", -1))
         capi.check(L.dq_set_tuning(b"evo_vec", -1))
+        capi.check(L.dq_set_tuning(b"evo_diag_batch", 0))
+        capi.check(L.dq_set_tuning(b"evo_lanes", 4))
     chosen = B.tune()
-    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) and w in (0, 1, 2)
+    assert len(chosen) == B.n_classes and all(r in (1, 2, 4, 8) and x in (0, 1) and w in (0, 1, 2, 3, 4)
                                               for r, x, w in chosen)
     for a, b in zip(run(), base):
         np.testing.assert_array_equal(a, b)
