@@ -8,8 +8,11 @@
 // d = c - r = sum over the flipped qubits of +-2^k, so a term with w flips can touch 2^w diagonals; the result keeps
 // exactly that structural set (ascending; a diagonal whose terms cancel stays, all zero -- dq_prune drops it).
 //
-// One lane owns one row and adds the terms in order, so every element's sum has a fixed order and no two lanes write
-// the same element.
+// Each element is written once. The host lists, for every diagonal, the terms that can land on it (a term with w flipped
+// qubits lands on one of its 2^w diagonals in each row, chosen by the row's bits at those qubits), in term order.
+// A lane owns rows and, diagonal by diagonal, adds the listed terms whose entry in its row falls on that diagonal,
+// then stores the sum: the stores of a wave are coalesced, every stored element of every diagonal is written exactly
+// once (so the layout needs no zero fill beforehand), and each element sums its terms in term order.
 #include <algorithm>
 #include <vector>
 
@@ -21,30 +24,63 @@ typedef double d2v __attribute__((ext_vector_type(2)));  // one 16-byte access f
 
 namespace dq {
 
-__global__ void pauli_rows_kernel(int64_t N, int n_terms, const uint64_t *__restrict__ xm,
-                                  const uint64_t *__restrict__ zm, const double *__restrict__ cre,
-                                  const double *__restrict__ cim, int n_diags, const DiagMeta *__restrict__ meta,
-                                  double *__restrict__ re, double *__restrict__ im) {
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < N;
-         r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        for (int j = 0; j < n_terms; ++j) {
+constexpr int PAULI_RPT = 4;  // rows per lane
+
+__global__ __launch_bounds__(256) void pauli_diag_kernel(int64_t N, int n_diags, const DiagMeta *__restrict__ meta,
+                                                         const int32_t *__restrict__ list_start,
+                                                         const int32_t *__restrict__ list_term,
+                                                         const uint64_t *__restrict__ xm,
+                                                         const uint64_t *__restrict__ zm,
+                                                         const double *__restrict__ cre,
+                                                         const double *__restrict__ cim, double *__restrict__ re,
+                                                         double *__restrict__ im) {
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * (256 * PAULI_RPT) + threadIdx.x;
+    for (int k = blockIdx.y; k < n_diags; k += gridDim.y) {  // uniform
+        const DiagMeta md = meta[k];
+        const int t0 = list_start[k], t1 = list_start[k + 1];
+        double ar[PAULI_RPT], ai[PAULI_RPT];
+#pragma unroll
+        for (int q = 0; q < PAULI_RPT; ++q) { ar[q] = 0.0; ai[q] = 0.0; }
+        for (int t = t0; t < t1; ++t) {  // uniform
+            const int j = list_term[t];
             const uint64_t x = xm[j], z = zm[j];
-            const int64_t c = static_cast<int64_t>(static_cast<uint64_t>(r) ^ x);
-            const int64_t d = c - r;
-            int lo = 0, hi = n_diags - 1;  // the slot of offset d (offsets ascend; d is always present)
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (meta[mid].off < d) lo = mid + 1;
-                else hi = mid;
+            const int ny = __popcll(x & z) & 3;  // i^ny from the term's Y factors
+            const double pr0 = ny == 0 ? 1.0 : (ny == 2 ? -1.0 : 0.0);
+            const double pi0 = ny == 1 ? 1.0 : (ny == 3 ? -1.0 : 0.0);
+            const double vr = cre[j] * pr0 - cim[j] * pi0, vi = cre[j] * pi0 + cim[j] * pr0;
+#pragma unroll
+            for (int q = 0; q < PAULI_RPT; ++q) {
+                const int64_t r = r0 + 256 * q;
+                const int64_t c = static_cast<int64_t>(static_cast<uint64_t>(r) ^ x);
+                if (r < N && c - r == md.off) {
+                    const bool neg = __popcll(static_cast<uint64_t>(c) & z) & 1;
+                    ar[q] += neg ? -vr : vr;
+                    ai[q] += neg ? -vi : vi;
+                }
             }
-            const int ny = __popcll(x & z) & 3;
-            double pr = ny == 0 ? 1.0 : (ny == 2 ? -1.0 : 0.0);
-            double pi = ny == 1 ? 1.0 : (ny == 3 ? -1.0 : 0.0);
-            if (__popcll(static_cast<uint64_t>(c) & z) & 1) { pr = -pr; pi = -pi; }
-            const int64_t idx = meta[lo].rowbase + r;
-            re[idx] += cre[j] * pr - cim[j] * pi;
-            im[idx] += cre[j] * pi + cim[j] * pr;
         }
+#pragma unroll
+        for (int q = 0; q < PAULI_RPT; ++q) {
+            const int64_t r = r0 + 256 * q;
+            if (r >= md.rlo && r < md.rhi) {
+                re[md.rowbase + r] = ar[q];
+                im[md.rowbase + r] = ai[q];
+            }
+        }
+    }
+}
+
+// The diagonals of term j: every +-2^k combination over its flipped qubits (ascending order not implied).
+static void term_offsets(int n_qubits, uint64_t x, std::vector<int64_t> &out) {
+    std::vector<int> bits;
+    for (int k = 0; k < n_qubits; ++k)
+        if ((x >> k) & 1) bits.push_back(k);
+    if (bits.size() > 20) throw ArgError("pauli_sum_device: a term flips more than 20 qubits (2^20+ diagonals)");
+    const uint64_t combos = uint64_t(1) << bits.size();
+    for (uint64_t s = 0; s < combos; ++s) {
+        int64_t d = 0;
+        for (size_t b = 0; b < bits.size(); ++b) d += ((s >> b) & 1) ? -(int64_t(1) << bits[b]) : (int64_t(1) << bits[b]);
+        out.push_back(d);
     }
 }
 
@@ -52,16 +88,7 @@ __global__ void pauli_rows_kernel(int64_t N, int n_terms, const uint64_t *__rest
 static std::vector<int32_t> pauli_offsets(int n_qubits, int n_terms, const uint64_t *xm) {
     std::vector<int64_t> all;
     for (int j = 0; j < n_terms; ++j) {
-        std::vector<int> bits;
-        for (int k = 0; k < n_qubits; ++k)
-            if ((xm[j] >> k) & 1) bits.push_back(k);
-        if (bits.size() > 20) throw ArgError("pauli_sum_device: a term flips more than 20 qubits (2^20+ diagonals)");
-        const uint64_t combos = uint64_t(1) << bits.size();
-        for (uint64_t s = 0; s < combos; ++s) {
-            int64_t d = 0;
-            for (size_t b = 0; b < bits.size(); ++b) d += ((s >> b) & 1) ? -(int64_t(1) << bits[b]) : (int64_t(1) << bits[b]);
-            all.push_back(d);
-        }
+        term_offsets(n_qubits, xm[j], all);
         if (all.size() > (size_t(1) << 26)) {
             std::sort(all.begin(), all.end());
             all.erase(std::unique(all.begin(), all.end()), all.end());
@@ -71,6 +98,31 @@ static std::vector<int32_t> pauli_offsets(int n_qubits, int n_terms, const uint6
     all.erase(std::unique(all.begin(), all.end()), all.end());
     if (all.size() > (size_t(1) << 22)) throw ArgError("pauli_sum_device: more than 2^22 diagonals");
     return std::vector<int32_t>(all.begin(), all.end());
+}
+
+// Per diagonal slot, the terms that can land on it, in term order: list_start (n_diags + 1) and list_term.
+static void pauli_lists(int n_qubits, int n_terms, const uint64_t *xm, const std::vector<int32_t> &offs,
+                        std::vector<int32_t> &list_start, std::vector<int32_t> &list_term) {
+    std::vector<std::vector<int32_t>> slots_of(static_cast<size_t>(n_terms));
+    std::vector<int64_t> cnt(offs.size() + 1, 0);
+    std::vector<int64_t> ds;
+    for (int j = 0; j < n_terms; ++j) {
+        ds.clear();
+        term_offsets(n_qubits, xm[j], ds);
+        for (int64_t d : ds) {
+            const auto it = std::lower_bound(offs.begin(), offs.end(), static_cast<int32_t>(d));
+            const int32_t k = static_cast<int32_t>(it - offs.begin());
+            slots_of[j].push_back(k);
+            ++cnt[static_cast<size_t>(k) + 1];
+        }
+    }
+    for (size_t k = 0; k < offs.size(); ++k) cnt[k + 1] += cnt[k];
+    if (cnt.back() > 0x7fffffff) throw ArgError("pauli_sum_device: too many (term, diagonal) pairs");
+    list_start.assign(cnt.begin(), cnt.end());
+    list_term.assign(static_cast<size_t>(cnt.back()), 0);
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int j = 0; j < n_terms; ++j)  // terms in order, so each slot's list ascends
+        for (int32_t k : slots_of[j]) list_term[static_cast<size_t>(fill[k]++)] = j;
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -456,17 +508,33 @@ extern "C" int dq_devmat_from_pauli_sum(int n_qubits, int n_terms, const uint64_
         hipStream_t s = as_stream(stream);
         const int64_t N = static_cast<int64_t>(lim);
         const std::vector<int32_t> offs = pauli_offsets(n_qubits, n_terms, x_masks);
+        // every stored element is written by pauli_diag_kernel; only the alignment gaps are zeroed
         auto M = make_devmat(static_cast<uint32_t>(N), static_cast<uint32_t>(N), offs.data(),
-                             static_cast<int>(offs.size()), DQ_F64, s);  // zeroed
-        if (n_terms && !offs.empty()) {
-            DevBuf dx(8 * n_terms), dz(8 * n_terms), dr(8 * n_terms), di(8 * n_terms);
-            DQ_HIP(hipMemcpyAsync(dx.p, x_masks, dx.bytes, hipMemcpyHostToDevice, s));
-            DQ_HIP(hipMemcpyAsync(dz.p, z_masks, dz.bytes, hipMemcpyHostToDevice, s));
-            DQ_HIP(hipMemcpyAsync(dr.p, coef_re, dr.bytes, hipMemcpyHostToDevice, s));
-            DQ_HIP(hipMemcpyAsync(di.p, coef_im, di.bytes, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(pauli_rows_kernel, dim3(blocks_for(N, 256, 1 << 16)), dim3(256), 0, s, N, n_terms,
-                               dx.as<uint64_t>(), dz.as<uint64_t>(), dr.as<double>(), di.as<double>(), M->n(),
-                               M->d_meta.as<DiagMeta>(), M->re.as<double>(), M->im.as<double>());
+                             static_cast<int>(offs.size()), DQ_F64, s, false);
+        zero_gaps(*M, s);
+        if (!offs.empty()) {
+            std::vector<int32_t> list_start, list_term;
+            pauli_lists(n_qubits, n_terms, x_masks, offs, list_start, list_term);
+            const int nt = std::max(1, n_terms);
+            DevBuf dx(8 * nt), dz(8 * nt), dr(8 * nt), di(8 * nt), dls(4 * list_start.size()),
+                dlt(4 * std::max<size_t>(1, list_term.size()));
+            if (n_terms) {
+                DQ_HIP(hipMemcpyAsync(dx.p, x_masks, 8 * n_terms, hipMemcpyHostToDevice, s));
+                DQ_HIP(hipMemcpyAsync(dz.p, z_masks, 8 * n_terms, hipMemcpyHostToDevice, s));
+                DQ_HIP(hipMemcpyAsync(dr.p, coef_re, 8 * n_terms, hipMemcpyHostToDevice, s));
+                DQ_HIP(hipMemcpyAsync(di.p, coef_im, 8 * n_terms, hipMemcpyHostToDevice, s));
+            }
+            DQ_HIP(hipMemcpyAsync(dls.p, list_start.data(), dls.bytes, hipMemcpyHostToDevice, s));
+            if (!list_term.empty())
+                DQ_HIP(hipMemcpyAsync(dlt.p, list_term.data(), 4 * list_term.size(), hipMemcpyHostToDevice, s));
+            const int64_t chunks = (N + 256 * PAULI_RPT - 1) / (256 * PAULI_RPT);
+            if (chunks > 0x7fffffff) throw ArgError("pauli_sum_device: too many rows");
+            // enough workgroups to fill the GPU: slots are spread over the grid's y when the rows are few
+            const int64_t gy = std::min<int64_t>(M->n(), std::max<int64_t>(1, 4096 / chunks));
+            hipLaunchKernelGGL(pauli_diag_kernel, dim3(static_cast<unsigned>(chunks), static_cast<unsigned>(gy)),
+                               dim3(256), 0, s, N, M->n(), M->d_meta.as<DiagMeta>(), dls.as<int32_t>(),
+                               dlt.as<int32_t>(), dx.as<uint64_t>(), dz.as<uint64_t>(), dr.as<double>(),
+                               di.as<double>(), M->re.as<double>(), M->im.as<double>());
             DQ_LAUNCH_CHECK();
             DQ_HIP(hipStreamSynchronize(s));  // the term arrays are freed on return
         }
