@@ -182,7 +182,7 @@ int dq_release_cached_memory(void);
  * diagonals within half a tile) and "spmv_xcd" (-1 = automatic, 0 = row tiles round-robin over the XCDs, 1 = each
  * XCD a contiguous eighth of the tiles) override a matrix's tuned SpMV schedule; "product_interleave" (0, 1 = default: row
  * chunks of a product interleaved over the XCDs) and "product_nt_store" (0, 1 = default: non-temporal stores of the
- * product's values), "product_rows_per_lane" (4, 8 = default, 16: row chunks of 256 x that) and
+ * product's values), "product_rows_per_lane" (0 = default: 4 below 2^16 rows, else 8; or 4, 8, 16: row chunks of 256 x that) and
  * "product_skip_zero_blocks" (1 = default: for operands of at least 2^16 rows, map their all-zero 256-row blocks and
  * drop the pairs whose rows in a chunk are all zero, same bits; 0 = read every pair) set the SpGEMM kernel's schedule; "evo_rows_per_lane" (0 = tuned, or 1, 2, 4, 8), "evo_xcd"
  * (-1 = tuned, 0, 1), "evo_lds" (-1 = tuned, 0, 1) and "evo_vec" (-1 = tuned, 0 = a lane per state, 1 = four states

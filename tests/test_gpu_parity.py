@@ -456,7 +456,7 @@ def test_product_zero_block_skip_is_exact(d, model):
                     np.testing.assert_array_equal(tuned[k], base["HH"][k], err_msg=f"tuned map {k}")
     finally:
         capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 1))
-        capi.check(L.dq_set_tuning(b"product_rows_per_lane", 8))
+        capi.check(L.dq_set_tuning(b"product_rows_per_lane", 0))
     with pytest.raises(RuntimeError, match="product_skip_zero_blocks"):
         capi.check(L.dq_set_tuning(b"product_skip_zero_blocks", 2))
 
