@@ -914,8 +914,8 @@ def batched_section(args, torch, d, capi, Hm, quick=False):
     load()
     B.set_schedule([j["t"] for j in jobs], [m for m, s in plain], [s for m, s in plain])
     tuned = B.tune()
-    log(f"[bench] batched tuned (rows per lane, XCD order, form: 0 lane/state, 1 +LDS, 2 four states/lane, 3/4 +2/4 diagonals per load batch) per "
-        f"class: {tuned}")
+    log("[bench] batched tuned (rows per lane, XCD order, form: 0 lane/state, 1 +LDS, 2 four states/lane, 3/4 +2/4 "
+        f"diagonals per load batch) per class: {tuned}")
     load()
     learned_run()  # warm-up: builds the graph once
     load()

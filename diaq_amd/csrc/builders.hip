@@ -8,8 +8,9 @@
 // d = c - r = sum over the flipped qubits of +-2^k, so a term with w flips can touch 2^w diagonals; the result keeps
 // exactly that structural set (ascending; a diagonal whose terms cancel stays, all zero -- dq_prune drops it).
 //
-// Each element is written once. The host lists, for every diagonal, the terms that can land on it (a term with w flipped
-// qubits lands on one of its 2^w diagonals in each row, chosen by the row's bits at those qubits), in term order.
+// Each element is written once. The host lists, for every diagonal, the terms that can land on it (a term with w
+// flipped qubits lands on one of its 2^w diagonals in each row, chosen by the row's bits at those qubits), in term
+// order.
 // A lane owns rows and, diagonal by diagonal, adds the listed terms whose entry in its row falls on that diagonal,
 // then stores the sum: the stores of a wave are coalesced, every stored element of every diagonal is written exactly
 // once (so the layout needs no zero fill beforehand), and each element sums its terms in term order.

@@ -250,8 +250,9 @@ __global__ __launch_bounds__(256) void horner4_kernel(int64_t N, int n, const Di
         }
 #pragma unroll
         for (int u = 0; u < DU; ++u) {
+            // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) {  // rows off the diagonal add nothing (a compensated sum is not inert to zeros)
+            for (int q = 0; q < RPT; ++q) {
                 const int64_t r = tile + RPP * q + WR * wave + rw;
                 if (((lv[u] >> (4 * q + wave)) & 1u) && r >= lo_[u] && r < hi_[u]) {
                     cmac(hr[q][0], hi[q][0], va_r[u][q], va_i[u][q], static_cast<TV>(vx_r[u][q].x),
