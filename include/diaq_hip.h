@@ -390,12 +390,12 @@ int dq_evo_tune(dq_evo *E, int *chosen_rpt, int *chosen_xcd, int *chosen_lds);
  * first sub-step's Taylor terms, the largest over the job's states. Leaves the states unchanged. */
 int dq_evo_probe(dq_evo *E, double *est);
 
-/* Evolve every job by its schedule; use_graph != 0 captures the whole batch (one stream per class) into a HIP graph
- * on the first call and replays it; renormalize != 0 ends with an FP64 renormalisation of every state. ms_out (may
- * be NULL) receives the GPU time. The graph is captured again only when what its launches depend on changed: a
- * class's number of Horner levels (max over its jobs of m 2^s), the tuned schedule, the evo_* tuning keys or
- * renormalize. A new schedule that keeps those replays the same graph (the kernels read t, m and s from device
- * memory). */
+/* Evolve every job by its schedule; the classes run on evo_lanes streams (see dq_set_tuning), forked from and joined
+ * to `stream`. use_graph != 0 captures the whole batch into a HIP graph on the first call and replays it;
+ * renormalize != 0 ends with an FP64 renormalisation of every state. ms_out (may be NULL) receives the GPU time. The
+ * graph is captured again only when what its launches depend on changed: a class's number of Horner levels (max over
+ * its jobs of m 2^s), the tuned schedule, the evo_* tuning keys, the classes' assignment to streams or renormalize.
+ * A new schedule that keeps those replays the same graph (the kernels read t, m and s from device memory). */
 int dq_evo_run(dq_evo *E, int use_graph, int renormalize, uintptr_t stream, float *ms_out);
 
 /* ------------------------------------------------------------------------------------------------------------------
