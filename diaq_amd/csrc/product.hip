@@ -6,8 +6,8 @@
 // Dry-run (host, integer only): every pair (slot a, slot b) with dC = dA + dB in [-(rows-1), cols-1] contributes to
 // result diagonal dC; result offsets ascend and each dC keeps its pairs in A-slot order (then B-slot order), so the
 // sums run in the reference's order. The pairs are bucketed by dC with a counting sort (two passes over the slot
-// pairs, in slot order, so each bucket keeps the enumeration order); offset ranges too wide for a count array fall
-// back to a stable sort.
+// pairs, in slot order, so each bucket keeps the enumeration order) when the offset range is at most four times the
+// pairs; fewer pairs over a wider range take a stable sort.
 //
 // Kernel: in row space the pair (a, b) adds A_a[i] * B_b[i + dA] to C_c[i] (row i of A, row i + dA of B), where a
 // diagonal's element for row r sits at rowbase + r; rows outside a diagonal's range contribute nothing, exactly the
@@ -50,7 +50,9 @@ static DryRun dryrun(uint32_t rows, uint32_t cols, const std::vector<int32_t> &o
         r.pair_start.push_back(0);
         return r;
     }
-    if (range <= std::max<int64_t>(int64_t(1) << 24, 4 * n_ab)) {  // counting sort by dC
+    // counting sort by dC when the offset range is at most a few times the pairs (a Taylor term's 8191 x 25 offsets,
+    // range 8191); a few pairs over a wide range (TFIM-20 H.H: 1681 pairs, range 2^21) sort faster than they count
+    if (range <= 4 * n_ab) {
         std::vector<int64_t> cnt(static_cast<size_t>(range), 0);
         for (size_t a = 0; a < oa.size(); ++a)
             for (size_t b = 0; b < ob.size(); ++b) {
